@@ -1,0 +1,121 @@
+/*
+ * torchff_hip.h — C ABI of the MI355X (gfx950) nonbonded hot path.
+ *
+ * Plain pointers and sizes only.  All array arguments are DEVICE pointers
+ * unless the name ends in _host; every call enqueues work on `stream` and
+ * returns without synchronising.  Kernels never allocate: each entry point
+ * takes a caller-provided device workspace whose size the matching
+ * *_workspace_bytes() query returns (so the calls are safe to capture in a
+ * HIP graph).  Return value 0 means success; otherwise tff_last_error()
+ * describes the failure (thread-local string).
+ *
+ * dtype: 0 = float32, 1 = float64 (coordinates, charges, parameters and
+ * outputs share it).  Index arrays are int64.  box_host: 9 float64 values,
+ * row-major lattice vectors (rows a, b, c) in host memory, or NULL for no
+ * periodicity where allowed.
+ *
+ * Each entry point replaces one operator of the reference library
+ * (THGLab/torchff-lib), cited by file:line.
+ */
+#ifndef TORCHFF_HIP_H
+#define TORCHFF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* the HIP stream type (hipStream_t) */
+typedef struct ihipStream_t* tff_stream_t;
+
+/* ------------------------------------------------------------------ misc */
+const char* tff_last_error(void);
+int tff_abi_version(void);
+
+/* Kernel timing hooks (no counterpart in the reference; instrumentation for
+ * benchmarks).  When enabled, the main kernel of each entry point is bracketed
+ * by HIP events on the caller's stream; tff_prof_collect waits for and sums the
+ * recorded launches whose name matches (NULL: all), then forgets them.
+ * Names: "nblist_cell_pairs", "nblist_nsquared", "pair_terms", "pme_spread",
+ * "pme_fft", "pme_fft_conv", "pme_gather". */
+void tff_prof_enable(int on);
+int tff_prof_collect(const char* name, double* total_ms, long long* count);
+
+/* --------------------------------------------------------- neighbor list */
+#define TFF_NBLIST_NSQUARED 0
+#define TFF_NBLIST_CELL_LIST 1
+
+size_t tff_nblist_workspace_bytes(long long natoms, const double* box_host, double cutoff, int algorithm);
+
+/* Replaces torch.ops.torchff.build_neighbor_list_nsquared(_out) and
+ * build_neighbor_list_cell_list(_out)
+ * (csrc/nblist/nblist_interface.cpp:5-8; kernels csrc/nblist/nblist_nsquared_cuda.cu:29-198,
+ * csrc/nblist/nblist_clist_cuda.cu:44-525; Python caller torchff/nblist.py:142-160).
+ * pairs: (max_npairs, 2) int64 output, rows (a, b) with a > b, unused rows -1.
+ * npairs: one int32, the number of pairs found (may exceed max_npairs: then
+ * only the first max_npairs were stored).  excl_row_ptr / excl_col: CSR
+ * exclusions (torchff/nblist.py:50-123) or both NULL.  The cell-list
+ * algorithm requires box_host.  Pair predicate: float64 |minimg(dr)|^2 <= cutoff^2. */
+int tff_build_neighbor_list(int dtype, const void* coords, long long natoms, const double* box_host,
+                            double cutoff, int algorithm, int include_self, const int64_t* excl_row_ptr,
+                            const int64_t* excl_col, int64_t* pairs, long long max_npairs, int32_t* npairs,
+                            void* workspace, size_t workspace_bytes, tff_stream_t stream);
+
+/* ------------------------------------------------------------ pair terms */
+#define TFF_PAIR_NONBONDED 0   /* shifted Coulomb + LB Lennard-Jones */
+#define TFF_PAIR_EWALD_LJ 1    /* erfc Coulomb + LB Lennard-Jones + exclusion correction */
+#define TFF_PAIR_COUL_EWALD 2  /* erfc Coulomb (rank-0 multipoles) + exclusion correction */
+#define TFF_PAIR_LJ_TYPES 3    /* Lennard-Jones from a (T, T) type table */
+#define TFF_PAIR_LJ_PAIRS 4    /* Lennard-Jones from per-pair sigma / epsilon */
+
+size_t tff_pair_workspace_bytes(int mode, long long natoms, long long npairs, long long ntypes);
+
+/* Replaces torch.ops.torchff.compute_nonbonded_energy_from_atom_pairs and
+ * compute_nonbonded_forces_from_atom_pairs (csrc/nonbonded/nonbonded_interface.cpp:9-10,
+ * kernel csrc/nonbonded/nonbonded_atom_pairs_cuda.cu:16-305) in mode NONBONDED;
+ * compute_multipolar_energy_from_atom_pairs for rank 0
+ * (csrc/multipoles/multipoles_interface.cpp:6, csrc/multipoles/multipoles.cu:24-152) in mode
+ * COUL_EWALD; compute_lennard_jones_energy (csrc/vdw/vdw_interface.cpp:6, csrc/vdw/vdw.cu:82-195)
+ * in modes LJ_TYPES / LJ_PAIRS; and the fused erfc-Coulomb + LJ extension in mode EWALD_LJ.
+ *
+ * Outputs (any may be NULL): energy_out (1 value) = scale * E;
+ * grad_* = scale * dE/d(argument), written (not accumulated); for LJ_TYPES
+ * grad_sigma / grad_epsilon have ntypes^2 entries, for LJ_PAIRS npairs entries,
+ * otherwise natoms; forces_accum += -scale * dE/dcoords.
+ * pairs_excl are used only by the Ewald modes with ewald_alpha >= 0. */
+int tff_pair_energy(int dtype, int mode, const void* coords, long long natoms, const double* box_host,
+                    const int64_t* pairs, long long npairs, const int64_t* pairs_excl, long long npairs_excl,
+                    const void* charges, const void* sigma, const void* epsilon, const int64_t* atom_types,
+                    long long ntypes, double coul_constant, double cutoff, double ewald_alpha, int do_shift,
+                    double scale, void* energy_out, void* grad_coords, void* grad_charges, void* grad_sigma,
+                    void* grad_epsilon, void* forces_accum, void* workspace, size_t workspace_bytes,
+                    tff_stream_t stream);
+
+/* ------------------------------------------------------------------- PME */
+size_t tff_pme_workspace_bytes(int dtype, long long natoms, int K0, int K1, int K2);
+
+/* Replaces the forward of torch.ops.torchff.pme_long_range for point charges
+ * (csrc/pme/pme_interface.cpp:6-9, csrc/pme/pme.cu:19-168), any spline order 3..8.
+ * Writes energy_out = scale * (E_recip + E_self) and the potential grid
+ * phi_grid_out (K0*K1*K2 values, z fastest) that tff_pme_gather consumes.
+ * xmoduli (K0), ymoduli (K1), zmoduli (K2, or K2/2+1 when zmoduli_half) hold
+ * |theta(k)| per axis (torchff/bsplines.py:57-107 for order 6). */
+int tff_pme_energy(int dtype, const void* coords, const void* charges, long long natoms, const double* box_host,
+                   int K0, int K1, int K2, int order, double alpha, const void* xmoduli, const void* ymoduli,
+                   const void* zmoduli, int zmoduli_half, double scale, void* energy_out, void* phi_grid_out,
+                   void* workspace, size_t workspace_bytes, tff_stream_t stream);
+
+/* Replaces the backward of pme_long_range (csrc/pme/pme.cu:170-246, interpolate_kernel
+ * csrc/pme/kernels.cuh:191-470): epot_out = scale * dE/dq (the potential including the
+ * self term), dcoords_out = scale * dE/dcoords.  Either output may be NULL. */
+int tff_pme_gather(int dtype, const void* phi_grid, const void* coords, const void* charges, long long natoms,
+                   const double* box_host, int K0, int K1, int K2, int order, double alpha, double scale,
+                   void* epot_out, void* dcoords_out, tff_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCHFF_HIP_H */

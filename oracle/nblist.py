@@ -46,12 +46,14 @@ def coo_to_csr(pairs_coo, natoms):
 
 def _excluded_mask(i, j, row_ptr, col):
     """True where (i, j) is in the CSR exclusion set (exclusions.cuh:26-36)."""
-    if col is None or len(col) == 0:
+    if col is None or len(col) == 0 or len(i) == 0:
         return np.zeros(len(i), dtype=bool)
     natoms = len(row_ptr) - 1
     rows = np.repeat(np.arange(natoms, dtype=np.int64), np.diff(row_ptr))
-    ekeys = rows * natoms + col
-    return np.isin(i * natoms + j, ekeys)
+    ekeys = np.sort(rows * natoms + col)
+    keys = i * natoms + j
+    pos = np.minimum(np.searchsorted(ekeys, keys), len(ekeys) - 1)
+    return ekeys[pos] == keys
 
 
 def _predicate(coords, box, box_inv, cutoff2, i, j):
@@ -104,22 +106,29 @@ def brute_force(coords, box, cutoff, excl=None, include_self=False, chunk=2_000_
 
 
 def cell_search(coords, box, cutoff, excl=None, include_self=False):
-    """Same set as :func:`brute_force`, found through cubic-cell binning (periodic box).
+    """Same set as :func:`brute_force`, found through cell binning (periodic box).
 
-    Candidate generation only narrows the search (cells of edge >= cutoff along
-    each reciprocal direction, 27 neighbours); every candidate passes through
-    the identical canonical predicate.  Falls back to the brute force when
-    fewer than 3 cells fit along an axis.
+    Candidate generation only narrows the search: cells at least cutoff/reach
+    wide along each reciprocal direction (reach 2 when 5 cells fit, else 1 when
+    3 fit), a half shell of neighbour cells so every cell pair is visited once;
+    every candidate passes through the identical canonical predicate.  Falls
+    back to the brute force when fewer than 3 cells fit along an axis.
     """
     coords = np.asarray(coords, dtype=np.float64)
-    n = len(coords)
     box = np.asarray(box, dtype=np.float64)
     inv, _ = invert_box(box)
     invm = inv.reshape(3, 3)
-    # perpendicular widths of the cell: 1 / |column k of inv|
+    # perpendicular widths of the box: 1 / |column k of inv|
     widths = 1.0 / np.linalg.norm(invm, axis=0)
-    nc = np.floor(widths / (cutoff * (1 + 1e-9) + 1e-12)).astype(np.int64)
-    if np.any(nc < 3) or cutoff <= 0:
+    if cutoff <= 0:
+        return brute_force(coords, box, cutoff, excl, include_self)
+    reach = None
+    for R in (2, 1):
+        nc = np.floor(widths * R / (cutoff * (1 + 1e-9))).astype(np.int64)
+        if np.all(nc >= 2 * R + 1):
+            reach = R
+            break
+    if reach is None:
         return brute_force(coords, box, cutoff, excl, include_self)
     frac = coords @ invm
     frac -= np.floor(frac)
@@ -132,17 +141,17 @@ def cell_search(coords, box, cutoff, excl=None, include_self=False):
     cutoff2 = float(cutoff) * float(cutoff)
     row_ptr, col = (None, None) if excl is None else excl
     out_i, out_j = [], []
-    offs = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
+    side = 2 * reach + 1
+    offs = [(a, b, c) for a in range(-reach, reach + 1) for b in range(-reach, reach + 1)
+            for c in range(-reach, reach + 1)]
+    half = offs[len(offs) // 2:]  # (0,0,0) first, then the positive half shell
     cx, cy, cz = np.unravel_index(np.arange(ncell), tuple(nc))
-    for (a, b, c) in offs:
-        nx = (cx + a) % nc[0]
-        ny = (cy + b) % nc[1]
-        nz = (cz + c) % nc[2]
-        nid = (nx * nc[1] + ny) * nc[2] + nz
-        # candidate pairs: every atom of cell x every atom of neighbour cell
-        cnt_i = end - start
-        cnt_j = end[nid] - start[nid]
-        tot = cnt_i * cnt_j
+    cnt = end - start
+    for (a, b, c) in half:
+        same = (a, b, c) == (0, 0, 0)
+        nid = (((cx + a) % nc[0]) * nc[1] + (cy + b) % nc[1]) * nc[2] + (cz + c) % nc[2]
+        cnt_j = cnt[nid]
+        tot = cnt * cnt_j
         if tot.sum() == 0:
             continue
         cells = np.repeat(np.arange(ncell), tot)
@@ -151,19 +160,21 @@ def cell_search(coords, box, cutoff, excl=None, include_self=False):
         cj = cnt_j[cells]
         ia = order[start[cells] + k // cj]
         ja = order[start[nid[cells]] + k % cj]
-        sel = ia < ja if not include_self else ia <= ja
-        ia, ja = ia[sel], ja[sel]
+        if same:
+            sel = ia < ja if not include_self else ia <= ja
+            ia, ja = ia[sel], ja[sel]
         keep = _predicate(coords, box, inv, cutoff2, ia, ja)
         keep = np.where(ia == ja, include_self, keep)
+        ia, ja = ia[keep], ja[keep]
         if col is not None:
-            keep &= ~_excluded_mask(ia, ja, row_ptr, col)
-        out_i.append(ia[keep])
-        out_j.append(ja[keep])
-    i = np.concatenate(out_i)
-    j = np.concatenate(out_j)
-    pairs = _canonical(i, j)
-    # with nc >= 3 every neighbour cell is distinct, so no pair is found twice
-    return pairs
+            ex = _excluded_mask(np.minimum(ia, ja), np.maximum(ia, ja), row_ptr, col)
+            ia, ja = ia[~ex], ja[~ex]
+        out_i.append(ia)
+        out_j.append(ja)
+    del side
+    if not out_i:
+        return np.zeros((0, 2), np.int64)
+    return _canonical(np.concatenate(out_i), np.concatenate(out_j))
 
 
 def neighbor_pairs(coords, box, cutoff, exclusions=None, include_self=False):

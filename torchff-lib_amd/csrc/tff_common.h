@@ -1,0 +1,159 @@
+// Shared device/host helpers for the torchff MI355X (gfx950) library.
+//
+// Conventions used by every kernel in this library:
+//  * box: 3x3 row-major lattice vectors (rows a, b, c), as torchff passes it
+//    (csrc/common/pbc.cuh:6-18 of the reference); its inverse is formed on the
+//    host by cofactors in float64 (pbc.cuh:38-64) and travels by value.
+//  * minimum image: s = d . B^-1, s -= round(s), d = s . B, evaluated in
+//    float64 with floating-point contraction disabled, so the cutoff
+//    predicate is bit-identical to a plain IEEE evaluation on the CPU.
+//  * wavefronts are 64 lanes (CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TFF_WAVE 64
+
+enum TffDtype { TFF_F32 = 0, TFF_F64 = 1 };
+
+enum TffStatus {
+  TFF_OK = 0,
+  TFF_ERR_ARG = 1,
+  TFF_ERR_WORKSPACE = 2,
+  TFF_ERR_LAUNCH = 3,
+  TFF_ERR_UNSUPPORTED = 4,
+};
+
+struct TffBox {
+  double b[9];
+  double inv[9];
+  double det;
+  int periodic;
+};
+
+// Host: cofactor inverse with the reference's formula, contraction off.
+static inline TffBox tff_make_box(const double* m) {
+#pragma clang fp contract(off)
+  TffBox B;
+  for (int k = 0; k < 9; ++k) B.b[k] = m ? m[k] : 0.0;
+  B.periodic = m != nullptr;
+  if (!m) {
+    for (int k = 0; k < 9; ++k) B.inv[k] = 0.0;
+    B.det = 0.0;
+    return B;
+  }
+  double m00 = m[0], m01 = m[1], m02 = m[2];
+  double m10 = m[3], m11 = m[4], m12 = m[5];
+  double m20 = m[6], m21 = m[7], m22 = m[8];
+  double c00 = m11 * m22 - m12 * m21;
+  double c01 = m12 * m20 - m10 * m22;
+  double c02 = m10 * m21 - m11 * m20;
+  double c10 = m02 * m21 - m01 * m22;
+  double c11 = m00 * m22 - m02 * m20;
+  double c12 = m01 * m20 - m00 * m21;
+  double c20 = m01 * m12 - m02 * m11;
+  double c21 = m02 * m10 - m00 * m12;
+  double c22 = m00 * m11 - m01 * m10;
+  double det = m00 * c00 + m01 * c01 + m02 * c02;
+  double id = 1.0 / det;
+  B.inv[0] = c00 * id; B.inv[1] = c10 * id; B.inv[2] = c20 * id;
+  B.inv[3] = c01 * id; B.inv[4] = c11 * id; B.inv[5] = c21 * id;
+  B.inv[6] = c02 * id; B.inv[7] = c12 * id; B.inv[8] = c22 * id;
+  B.det = det;
+  return B;
+}
+
+// Minimum-image displacement, float64, fixed operation order, no FMA.
+__device__ __forceinline__ void tff_min_image_d(const TffBox& B, double& x, double& y, double& z) {
+#pragma clang fp contract(off)
+  if (!B.periodic) return;
+  double sa = x * B.inv[0] + y * B.inv[3] + z * B.inv[6];
+  double sb = x * B.inv[1] + y * B.inv[4] + z * B.inv[7];
+  double sc = x * B.inv[2] + y * B.inv[5] + z * B.inv[8];
+  sa = sa - round(sa);
+  sb = sb - round(sb);
+  sc = sc - round(sc);
+  x = sa * B.b[0] + sb * B.b[3] + sc * B.b[6];
+  y = sa * B.b[1] + sb * B.b[4] + sc * B.b[7];
+  z = sa * B.b[2] + sb * B.b[5] + sc * B.b[8];
+}
+
+__device__ __forceinline__ double tff_dist2_d(const TffBox& B, double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  tff_min_image_d(B, dx, dy, dz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Minimum image in the working precision T (used by energy kernels, not by predicates).
+template <typename T>
+__device__ __forceinline__ void tff_min_image(const TffBox& B, T& x, T& y, T& z) {
+  if (!B.periodic) return;
+  T sa = x * (T)B.inv[0] + y * (T)B.inv[3] + z * (T)B.inv[6];
+  T sb = x * (T)B.inv[1] + y * (T)B.inv[4] + z * (T)B.inv[7];
+  T sc = x * (T)B.inv[2] + y * (T)B.inv[5] + z * (T)B.inv[8];
+  sa -= rint(sa);
+  sb -= rint(sb);
+  sc -= rint(sc);
+  x = sa * (T)B.b[0] + sb * (T)B.b[3] + sc * (T)B.b[6];
+  y = sa * (T)B.b[1] + sb * (T)B.b[4] + sc * (T)B.b[7];
+  z = sa * (T)B.b[2] + sb * (T)B.b[5] + sc * (T)B.b[8];
+}
+
+// CSR exclusion test: is j listed in row i?  (reference csrc/nblist/exclusions.cuh:26-36)
+__device__ __forceinline__ bool tff_is_excluded(int64_t i, int64_t j, const int64_t* row_ptr,
+                                                const int64_t* col) {
+  if (!col) return false;
+  int64_t s = row_ptr[i], e = row_ptr[i + 1];
+  for (int64_t k = s; k < e; ++k)
+    if (col[k] == j) return true;
+  return false;
+}
+
+template <typename T>
+__device__ __forceinline__ T tff_wave_sum(T v) {
+  for (int off = TFF_WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, TFF_WAVE);
+  return v;
+}
+
+// Block-wide sum of a double; result valid in thread 0.  blockDim.x <= 1024.
+__device__ __forceinline__ double tff_block_sum(double v) {
+  __shared__ double s_part[1024 / TFF_WAVE];
+  v = tff_wave_sum(v);
+  int lane = threadIdx.x & (TFF_WAVE - 1);
+  int wid = threadIdx.x / TFF_WAVE;
+  __syncthreads();
+  if (lane == 0) s_part[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + TFF_WAVE - 1) / TFF_WAVE;
+    for (int w = 0; w < nw; ++w) r += s_part[w];
+  }
+  return r;
+}
+
+// Fixed-point (int64) accumulation of forces and other per-atom sums: value * 2^FP_SHIFT.
+#define TFF_FP_SCALE 4294967296.0  // 2^32
+__device__ __forceinline__ unsigned long long tff_to_fixed(double v) {
+  return (unsigned long long)(long long)__double2ll_rn(v * TFF_FP_SCALE);
+}
+__device__ __forceinline__ double tff_from_fixed(long long v) { return (double)v / TFF_FP_SCALE; }
+
+static inline size_t tff_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Thread-local error message of the last failing call.
+void tff_set_error(const char* fmt, ...);
+
+// Optional event timing around one kernel launch (no-ops unless tff_prof_enable(1)).
+void* tff_prof_begin(const char* name, hipStream_t stream);
+void tff_prof_end(void* token, hipStream_t stream);
+
+#define TFF_CHECK_LAUNCH(name)                                              \
+  do {                                                                      \
+    hipError_t _e = hipGetLastError();                                      \
+    if (_e != hipSuccess) {                                                 \
+      tff_set_error("%s: kernel launch failed: %s", name, hipGetErrorString(_e)); \
+      return TFF_ERR_LAUNCH;                                                \
+    }                                                                       \
+  } while (0)

@@ -1,0 +1,20 @@
+"""torchff nonbonded hot path for AMD Instinct MI355X (gfx950).
+
+Drop-in for the ``use_customized_ops=True`` path of THGLab/torchff-lib's
+neighbor list, fused nonbonded / rank-0 Ewald real-space / Lennard-Jones pair
+terms and smooth PME: the same module classes, functional wrappers and
+``torch.ops.torchff.*`` operators, computed by hand-written HIP kernels in
+``libtorchff_hip.so`` (C ABI declared in include/torchff_hip.h).
+"""
+
+from . import _ops  # noqa: F401  (registers torch.ops.torchff.*)
+from .pbc import PBC, applyPBC  # noqa: F401
+from .nblist import NeighborList, build_neighbor_list_nsquared, build_neighbor_list_cell_list  # noqa: F401
+from .nonbonded import (  # noqa: F401
+    Nonbonded, compute_nonbonded_energy_from_atom_pairs, compute_nonbonded_forces_from_atom_pairs,
+    compute_nonbonded_ewald_energy_from_atom_pairs,
+)
+from .multipoles import MultipolarInteraction  # noqa: F401
+from .vdw import Vdw, compute_lennard_jones_energy  # noqa: F401
+from .pme import PME  # noqa: F401
+from . import bsplines  # noqa: F401
