@@ -1,14 +1,21 @@
 """Energy + force evaluations per second on the 96,000-atom PME water box (one MI355X).
 
 One step = one full evaluation of the TIP3P nonbonded energy and forces of
-32,000 rigid waters (L = 9.86 nm): cell-list neighbor list rebuilt at
-rc + skin = 1.1 nm with the intramolecular exclusions, fused real-space
-erfc-Coulomb + Lennard-Jones with the exclusion correction at rc = 1.0 nm,
-order-5 PME on a 96^3 grid (spread, FFTs, influence function, gather), self
-energy; forces = -dE/dr.  Inputs are resident on the device before timing.
+32,000 rigid waters (L = 9.86 nm): spatial clustering / neighbour search rebuilt
+every call at rc + skin = 1.1 nm with the intramolecular exclusions,
+erfc-Coulomb + Lennard-Jones at rc = 1.0 nm with the (erfc - 1)/r exclusion
+correction, order-5 PME on a 96^3 grid (spread, FFTs, influence function,
+gather), self energy; forces = -dE/dr.  Inputs are resident on the device
+before timing.
 
-Prints one JSON line (see the end of main()).  ``--gpus`` above 1 is refused:
-this build runs on one GPU.
+Paths measured (same inputs):
+  mixed    FusedNonbondedPME(precision="mixed")  -> the headline value
+  fp64     FusedNonbondedPME(precision="fp64")
+  dropin   the reference-shaped modules in float64 (NeighborList cell_list with an
+           explicit (P, 2) pair list + Nonbonded(ewald) + PME, autograd forces):
+           the plain FP64 drop-in the speed-up is quoted against.
+Errors are stated against one float64 CPU oracle evaluation, which is also the
+timed CPU baseline.  Prints one JSON line.  --gpus above 1 is refused.
 """
 
 import argparse
@@ -30,56 +37,67 @@ import torch  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-PAIR_FLOPS = 60  # per in-cutoff pair: fixed tally (min-image 9, r^2 5, rsqrt 1, erfc+exp 10, Coulomb 4, LJ 8,
-#                 force scalar 8, force vector 3, accumulations 12)
+# floating-point operations per in-cutoff pair, a fixed tally: minimum image 9, r^2 5, rsqrt 1,
+# erfc + exp 10, Coulomb 4, LJ 8, force scalar 8, force vector 3, accumulations 12
+PAIR_FLOPS = 60
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="water96k")
-    ap.add_argument("--dtype", default="float64", choices=["float64", "float32"])
+    ap.add_argument("--dropin-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-atoms", type=int, default=0,
-                    help="size of the CPU-baseline sample (0: the full workload)")
+    ap.add_argument("--only", default="", help="comma list of paths to run (mixed,fp64,dropin)")
     return ap.parse_args()
 
 
 class DropIn:
-    """The reference-shaped module composition (NeighborList + Nonbonded(ewald) + PME), autograd forces."""
+    """Reference-shaped modules (NeighborList + Nonbonded(ewald) + PME) in float64, autograd forces."""
 
-    def __init__(self, w, dtype, dev):
+    def __init__(self, w, dev):
         import torchff
         from torchff.workloads import COULOMB_KJMOL
+        dt = torch.float64
         self.k = COULOMB_KJMOL
-        self.w = w
         n = w["natoms"]
         self.cut = w["cutoff"]
         self.rl = w["cutoff"] + w["skin"]
-        self.x = torch.tensor(w["coords"], dtype=dtype, device=dev)
-        self.box = torch.tensor(w["box"], dtype=dtype, device=dev)
-        self.q = torch.tensor(w["charges"], dtype=dtype, device=dev)
-        self.sig = torch.tensor(w["sigma"], dtype=dtype, device=dev)
-        self.eps = torch.tensor(w["epsilon"], dtype=dtype, device=dev)
+        self.x = torch.tensor(w["coords"], dtype=dt, device=dev)
+        self.box = torch.tensor(w["box"], dtype=dt, device=dev)
+        self.q = torch.tensor(w["charges"], dtype=dt, device=dev)
+        self.sig = torch.tensor(w["sigma"], dtype=dt, device=dev)
+        self.eps = torch.tensor(w["epsilon"], dtype=dt, device=dev)
         self.excl = torch.tensor(w["pairs_excl"], device=dev)
         self.nl = torchff.NeighborList(n, exclusions=torch.tensor(w["exclusions"], device=dev),
                                        algorithm="cell_list")
         self.cap = int(n * 4.0 / 3.0 * math.pi * self.rl ** 3 * 100.0 / 2.0 * 1.25)
         self.nb = torchff.Nonbonded()
-        self.pme = torchff.PME(w["alpha"], w["K"], 0, order=w["order"]).to(device=dev, dtype=dtype)
+        self.pme = torchff.PME(w["alpha"], w["K"], 0, order=w["order"]).to(device=dev, dtype=dt)
         self.alpha = w["alpha"]
-        self.npairs = None
 
     def step(self):
         x = self.x.detach().requires_grad_(True)
-        pairs, npairs = self.nl(x.detach(), self.box, self.rl, self.cap, padding=True)
-        self.npairs = npairs
+        pairs, _ = self.nl(x.detach(), self.box, self.rl, self.cap, padding=True)
         e = self.nb(x, pairs, self.box, self.sig, self.eps, self.q, self.k, self.cut, ewald_alpha=self.alpha,
                     pairs_excl=self.excl).sum() + self.pme(x, self.box, self.q) * self.k
         e.backward()
         return e.detach(), -x.grad
+
+
+class Fused:
+    def __init__(self, w, dev, precision):
+        from torchff.fused import for_workload
+        dt = torch.float64
+        self.m = for_workload(w, precision)
+        self.args = [torch.tensor(w[k], dtype=dt, device=dev) for k in ("coords", "box", "charges", "sigma",
+                                                                          "epsilon")]
+
+    def step(self):
+        E, F, _ = self.m(*self.args)
+        return E, F
 
 
 def time_steps(fn, steps, warmup):
@@ -94,22 +112,34 @@ def time_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, out
 
 
-def cpu_baseline(w, sample_atoms):
-    """Time the float64 CPU oracle (the reference's pure-PyTorch algorithms restated) on the host cores."""
-    from oracle import composite
-    from torchff.workloads import make_workload
-    threads = os.cpu_count() or 1
-    torch.set_num_threads(threads)
-    ws = w
-    if sample_atoms and sample_atoms < w["natoms"]:
-        ws = make_workload("water96k", n_waters=sample_atoms // 3)
-    t0 = time.perf_counter()
-    composite.energy_forces(ws)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"one float64 energy+force evaluation of {ws['natoms']} atoms "
-                      f"(cell-list pairs + autograd real space + PME), {dt:.1f} s on {threads} threads "
-                      f"of {platform.processor() or platform.machine()}"}
+def kernel_times(fn, steps, names):
+    from torchff import _lib
+    _lib.prof_collect()  # drop anything pending
+    _lib.prof_enable(True)
+    time_steps(fn, steps, 0)
+    _lib.prof_enable(False)
+    out = {}
+    for name in names:
+        tot, cnt = _lib.prof_collect(name)
+        if cnt:
+            out[name] = {"avg_ms": tot / cnt, "launches": cnt}
+    return out
+
+
+def pairs_within(w, dev, r):
+    """Number of non-excluded pairs within r (device neighbour list, float64 predicate)."""
+    import torchff
+    x = torch.tensor(w["coords"], dtype=torch.float64, device=dev)
+    box = torch.tensor(w["box"], dtype=torch.float64, device=dev)
+    nl = torchff.NeighborList(w["natoms"], exclusions=torch.tensor(w["exclusions"], device=dev),
+                              algorithm="cell_list")
+    cap = int(w["natoms"] * 4.0 / 3.0 * math.pi * r ** 3 * 100.0 / 2.0 * 1.3) + 1024
+    _, n = nl(x, box, r, cap, padding=True)
+    return int(n.item())
+
+
+def rms_rel(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
 
 
 def main():
@@ -117,45 +147,41 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
-    from torchff import _lib
     from torchff.workloads import make_workload
     dev = "cuda:0"
-    dtype = torch.float64 if args.dtype == "float64" else torch.float32
+    only = set(args.only.split(",")) if args.only else {"mixed", "fp64", "dropin"}
     w = make_workload(args.workload)
-    model = DropIn(w, dtype, dev)
-    sec, (E, F) = time_steps(model.step, args.steps, args.warmup)
-    npairs = int(model.npairs.item())
+    res = {}
+    outs = {}
+    if "mixed" in only:
+        f = Fused(w, dev, "mixed")
+        sec, out = time_steps(f.step, args.steps, args.warmup)
+        f.m.check()
+        res["mixed"] = sec
+        outs["mixed"] = (float(out[0]), out[1].double().cpu().numpy())
+        kern = kernel_times(f.step, args.steps,
+                            ["fused_cluster_pairs", "fused_tiles", "fused_spread", "fused_fft", "fused_gather"])
+    if "fp64" in only:
+        f64 = Fused(w, dev, "fp64")
+        sec, out = time_steps(f64.step, max(3, args.steps // 2), 2)
+        res["fp64"] = sec
+        outs["fp64"] = (float(out[0]), out[1].double().cpu().numpy())
+    if "dropin" in only:
+        d = DropIn(w, dev)
+        sec, out = time_steps(d.step, args.dropin_steps, 1)
+        res["dropin"] = sec
+        outs["dropin"] = (float(out[0]), out[1].double().cpu().numpy())
+    head = "mixed" if "mixed" in res else sorted(res)[0]
+    sec = res[head]
 
-    # per-kernel durations over a timed region of the same steps (events on the launch stream)
-    _lib.prof_enable(True)
-    time_steps(model.step, args.steps, 0)
-    _lib.prof_enable(False)
-    kern = {}
-    for name in ("nblist_cell_pairs", "pair_terms", "pme_spread", "pme_fft", "pme_fft_conv", "pme_gather"):
-        tot, cnt = _lib.prof_collect(name)
-        if cnt:
-            kern[name] = tot / cnt
-    # in-cutoff pairs (the algorithmic unit of the pair kernel)
-    import torchff  # noqa: F401
-    p_in = None
-    with torch.no_grad():
-        pairs, _ = model.nl(model.x, model.box, model.rl, model.cap, padding=True)
-        v = pairs[:npairs]
-        d = model.x[v[:, 0]] - model.x[v[:, 1]]
-        L = model.box.diagonal()
-        d = d - torch.round(d / L) * L
-        p_in = int(((d * d).sum(1) <= model.cut ** 2).sum().item())
-    dom = max(kern, key=kern.get)
-    peak = FP64_PEAK_TFLOPS if dtype == torch.float64 else FP32_PEAK_TFLOPS
-    if dom == "pair_terms":
-        achieved = p_in * PAIR_FLOPS / (kern[dom] * 1e-3) / 1e12
-        roof = {"kernel": dom, "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "avg_ms": kern[dom]}
-    else:
-        nbytes = npairs * 16 + w["natoms"] * 32
-        achieved = nbytes / (kern[dom] * 1e-3) / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_ms": kern[dom]}
+    p_in = pairs_within(w, dev, w["cutoff"])
+    roof = None
+    if head == "mixed":
+        t = kern["fused_tiles"]["avg_ms"]
+        achieved = p_in * PAIR_FLOPS / (t * 1e-3) / 1e12
+        roof = {"kernel": "fused_tiles", "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": t,
+                "units": f"{p_in} in-cutoff pairs x {PAIR_FLOPS} FLOP per launch"}
     out = {
         "metric": "energy+force evals/s, 96k-atom PME water",
         "value": 1.0 / sec,
@@ -167,17 +193,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if dtype == torch.float64 else "f32",
+        "dtype": "f32+i64" if head == "mixed" else ("f64" if head in ("fp64", "dropin") else head),
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {w['natoms']} atoms TIP3P, L={w['L']} nm, rc={w['cutoff']} "
-                               f"+ skin {w['skin']}, PME {w['K']}^3 order {w['order']}",
-                   "path": "drop-in modules (NeighborList cell_list + Nonbonded ewald + PME), autograd forces",
-                   "pairs_listed": npairs, "pairs_in_cutoff": p_in},
-        "kernels_ms": kern,
-        "roofline": roof,
+        "config": {"workload": f"{args.workload}: {w['natoms']}-atom rigid TIP3P box L={w['L']} nm, "
+                               f"rc={w['cutoff']} nm, list {w['cutoff'] + w['skin']} nm rebuilt per call, "
+                               f"PME {w['K']}^3 order {w['order']}, alpha={w['alpha']:.5f}",
+                   "path": {"mixed": "FusedNonbondedPME float32 arithmetic, int64 fixed-point accumulation",
+                            "fp64": "FusedNonbondedPME float64",
+                            "dropin": "drop-in modules float64"}[head],
+                   "pairs_in_cutoff": p_in},
+        "paths_evals_per_s": {k: 1.0 / v for k, v in res.items()},
     }
+    if "mixed" in res and "dropin" in res:
+        out["speedup_vs_fp64_dropin"] = res["dropin"] / res["mixed"]
+    if "mixed" in res and "fp64" in res:
+        out["speedup_mixed_vs_fused_fp64"] = res["fp64"] / res["mixed"]
+    if head == "mixed":
+        out["kernels_ms"] = {k: v["avg_ms"] for k, v in kern.items()}
+        out["roofline"] = roof
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_sample_atoms)
+        from oracle import composite
+        threads = os.cpu_count() or 1
+        torch.set_num_threads(threads)
+        t0 = time.perf_counter()
+        E_ref, F_ref, terms, _ = composite.energy_forces(w)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"one full float64 energy+force evaluation of the same {w['natoms']}-atom box by the CPU "
+                      f"oracle (numpy cell search + torch autograd real space + PME), {dt:.1f} s on {threads} "
+                      f"threads of {platform.processor() or platform.machine()}"}
+        scale = sum(abs(v) for v in terms.values())
+        out["errors_vs_fp64_oracle"] = {
+            k: {"energy_abs": abs(v[0] - E_ref), "energy_rel_to_sum_abs_terms": abs(v[0] - E_ref) / scale,
+                "force_rms_rel": rms_rel(v[1], F_ref)} for k, v in outs.items()}
     print(json.dumps(out))
 
 

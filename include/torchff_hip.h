@@ -39,7 +39,8 @@ int tff_abi_version(void);
  * by HIP events on the caller's stream; tff_prof_collect waits for and sums the
  * recorded launches whose name matches (NULL: all), then forgets them.
  * Names: "nblist_cell_pairs", "nblist_nsquared", "pair_terms", "pme_spread",
- * "pme_fft", "pme_fft_conv", "pme_gather". */
+ * "pme_fft", "pme_fft_conv", "pme_gather", "fused_cluster_pairs", "fused_tiles",
+ * "fused_spread", "fused_fft", "fused_gather". */
 void tff_prof_enable(int on);
 int tff_prof_collect(const char* name, double* total_ms, long long* count);
 
@@ -113,6 +114,32 @@ int tff_pme_energy(int dtype, const void* coords, const void* charges, long long
 int tff_pme_gather(int dtype, const void* phi_grid, const void* coords, const void* charges, long long natoms,
                    const double* box_host, int K0, int K1, int K2, int order, double alpha, double scale,
                    void* epot_out, void* dcoords_out, tff_stream_t stream);
+
+/* ------------------------------------------------- fused energy + forces */
+#define TFF_FUSED_PAIR_OVERFLOW 1  /* cluster-pair capacity exceeded: results incomplete */
+#define TFF_FUSED_EXCL_OVERFLOW 2  /* an atom has more than 8 exclusion partners */
+
+size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const double* box_host, double list_cutoff,
+                                 int K0, int K1, int K2);
+
+/* Energy and forces of the TIP3P-style composition in one call: the sum of
+ * compute_multipolar_energy_from_atom_pairs (rank 0, with the erfc - 1 correction on every
+ * excluded pair), compute_lennard_jones_energy with Lorentz-Berthelot mixing of per-atom
+ * sigma / epsilon, and prefactor * pme_long_range (any spline order 3..8), as composed by
+ * examples/tip3p/model.py:100-123, without materialising a pair list.  Orthorhombic box,
+ * each edge at least 2 * list_cutoff.  compute_dtype 0 = float32 arithmetic with int64
+ * fixed-point force / energy accumulation and float64 k-space sums, 1 = float64.
+ * energies_out (device, 4 doubles): Lennard-Jones, real-space Coulomb (incl. exclusion
+ * correction), reciprocal, self.  forces_out (io_dtype, N x 3) = -dE/dr.  status_out
+ * (device int32): TFF_FUSED_* flags, 0 when complete.  Moduli as for tff_pme_energy with
+ * zmoduli of length K2/2 + 1, in compute_dtype. */
+int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
+                            const void* sigma, const void* epsilon, long long natoms, const double* box_host,
+                            const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff, double list_cutoff,
+                            double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
+                            const void* xmoduli, const void* ymoduli, const void* zmoduli, double* energies_out,
+                            void* forces_out, int32_t* status_out, void* workspace, size_t workspace_bytes,
+                            tff_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,12 @@ def test_energy_forces_match_oracle(name):
     E, F, terms, pairs = composite.energy_forces(w)
     for dtype, te, tf in ((torch.float64, 1e-9, 1e-9), (torch.float32, 1e-5, 1e-4)):
         Ed, Fd, npairs = _device_energy_forces(w, dtype)
-        assert npairs == len(pairs)
+        if dtype == torch.float64:
+            assert npairs == len(pairs)
+        else:
+            # float32 coordinates move pairs within ~1e-7 relative of the list cutoff across it; the exact
+            # float32 set is checked against the oracle on the rounded values in test_gpu_nblist.py
+            assert abs(npairs - len(pairs)) <= 1e-6 * len(pairs) + 2
         # energy: relative to the sum of |terms| (the total is a near-cancellation)
         scale = sum(abs(v) for v in terms.values())
         assert abs(Ed - E) < te * scale, (dtype, Ed, E)

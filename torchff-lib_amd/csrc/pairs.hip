@@ -166,12 +166,10 @@ __global__ void k_zero_d(double* p, long long n) {
 }
 
 template <typename T>
-__global__ void k_sum_parts(const double* __restrict__ part, int n, double scale, T* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < n; ++k) s += part[k];
-    out[0] = (T)(s * scale);
-  }
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int n, double scale,
+                                                   T* __restrict__ out) {
+  double s = tff_sum_partials(part, n);
+  if (threadIdx.x == 0) out[0] = (T)(s * scale);
 }
 
 // dst = scale * src (accumulate: dst += scale * src)
@@ -258,7 +256,7 @@ int run(int mode, const T* coords, long long natoms, const TffBox& box, const lo
 #undef TFF_LAUNCH_PAIR
   tff_prof_end(tok, st);
   TFF_CHECK_LAUNCH("k_pair_terms");
-  if (energy) hipLaunchKernelGGL(k_sum_parts<T>, dim3(1), dim3(64), 0, st, w.e_part, (int)blocks, scale, energy);
+  if (energy) hipLaunchKernelGGL(k_sum_parts<T>, dim3(1), dim3(256), 0, st, w.e_part, (int)blocks, scale, energy);
   auto cast = [&](const double* s, long long n, double sc, int acc, T* d) {
     long long b = (n + 255) / 256;
     if (b > 4096) b = 4096;

@@ -1,0 +1,353 @@
+// Shared smooth-PME device code: B-spline weights, grid anchoring, mixed-radix
+// Stockham FFT passes over LDS-resident lines (with the fused influence-function
+// pass), used by the drop-in PME entry points and the fused cluster path.
+#pragma once
+
+#include <math.h>
+
+#include "tff_common.h"
+
+namespace tffpme {
+namespace {
+
+template <typename T>
+struct cx {
+  T x, y;
+};
+
+constexpr int FFT_THREADS = 256;
+constexpr int MAX_STAGES = 24;
+constexpr size_t FFT_LDS_BUDGET = 60 * 1024;  // dynamic LDS per workgroup, below the 64 KiB launch limit
+
+struct FftPlan {
+  int K;
+  int nst;
+  int radix[MAX_STAGES];
+};
+
+struct PmeGeom {
+  int K[3];
+  int order;
+  double alpha;
+  double inv[9];  // B^-1, row-major
+  double vol;
+};
+
+bool make_plan(int K, FftPlan& p) {
+  p.K = K;
+  p.nst = 0;
+  int m = K;
+  while (m % 4 == 0 && p.nst < MAX_STAGES) { p.radix[p.nst++] = 4; m /= 4; }
+  while (m % 2 == 0 && p.nst < MAX_STAGES) { p.radix[p.nst++] = 2; m /= 2; }
+  while (m % 3 == 0 && p.nst < MAX_STAGES) { p.radix[p.nst++] = 3; m /= 3; }
+  while (m % 5 == 0 && p.nst < MAX_STAGES) { p.radix[p.nst++] = 5; m /= 5; }
+  return m == 1 && K >= 1;
+}
+
+// ------------------------------------------------------------------ splines
+// v[k] = M_P(t + k), dv[k] = M_P'(t + k), k = 0..P-1, t in [0, 1)
+template <typename T, int P>
+__device__ __forceinline__ void spline_weights(T t, T* v, T* dv) {
+  T w[P];
+  w[0] = (T)1;
+#pragma unroll
+  for (int k = 1; k < P; ++k) w[k] = (T)0;
+#pragma unroll
+  for (int n = 2; n <= P; ++n) {
+    if (n == P) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) dv[k] = w[k] - (k > 0 ? w[k - 1] : (T)0);
+    }
+    const T div = (T)1 / (T)(n - 1);
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      T a = (k < n - 1) ? w[k] : (T)0;
+      T b = (k > 0) ? w[k - 1] : (T)0;
+      w[k] = ((t + (T)k) * a + ((T)n - t - (T)k) * b) * div;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) v[k] = w[k];
+}
+
+// grid base (already wrapped into [0, K)) and spline parameter t for one axis
+__device__ __forceinline__ void anchor_axis(double s, int K, int P, int& base, double& t) {
+  double shift = (P & 1) ? 0.5 : 0.0;
+  double m0 = ceil(s - shift);
+  t = m0 - s + 0.5 * P - (double)(P / 2);
+  long long b = (long long)m0 - P / 2;
+  b %= K;
+  if (b < 0) b += K;
+  base = (int)b;
+}
+
+// ------------------------------------------------------------------ FFT
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void butterfly(cx<T>* v) {
+  // X_k = sum_r v_r exp(DIR 2 pi i r k / R), DIR = -1 forward, +1 inverse
+  if constexpr (R == 2) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = {a.x + b.x, a.y + b.y};
+    v[1] = {a.x - b.x, a.y - b.y};
+  } else if constexpr (R == 4) {
+    cx<T> s02 = {v[0].x + v[2].x, v[0].y + v[2].y}, d02 = {v[0].x - v[2].x, v[0].y - v[2].y};
+    cx<T> s13 = {v[1].x + v[3].x, v[1].y + v[3].y}, d13 = {v[1].x - v[3].x, v[1].y - v[3].y};
+    // DIR * i * d13
+    cx<T> id13 = {-(T)DIR * d13.y, (T)DIR * d13.x};
+    v[0] = {s02.x + s13.x, s02.y + s13.y};
+    v[2] = {s02.x - s13.x, s02.y - s13.y};
+    v[1] = {d02.x + id13.x, d02.y + id13.y};
+    v[3] = {d02.x - id13.x, d02.y - id13.y};
+  } else if constexpr (R == 3) {
+    const T s3 = (T)0.86602540378443864676;
+    cx<T> t1 = {v[1].x + v[2].x, v[1].y + v[2].y};
+    cx<T> d = {v[1].x - v[2].x, v[1].y - v[2].y};
+    cx<T> m = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+    cx<T> id = {-(T)DIR * s3 * d.y, (T)DIR * s3 * d.x};
+    v[0] = {v[0].x + t1.x, v[0].y + t1.y};
+    v[1] = {m.x + id.x, m.y + id.y};
+    v[2] = {m.x - id.x, m.y - id.y};
+  } else {  // R == 5
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+    cx<T> a1 = {v[1].x + v[4].x, v[1].y + v[4].y}, b1 = {v[1].x - v[4].x, v[1].y - v[4].y};
+    cx<T> a2 = {v[2].x + v[3].x, v[2].y + v[3].y}, b2 = {v[2].x - v[3].x, v[2].y - v[3].y};
+    cx<T> x0 = v[0];
+    cx<T> m1 = {x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+    cx<T> m2 = {x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+    // n1 = DIR * i * (s1 b1 + s2 b2), n2 = DIR * i * (s2 b1 - s1 b2)
+    cx<T> p1 = {s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y};
+    cx<T> p2 = {s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y};
+    cx<T> n1 = {-(T)DIR * p1.y, (T)DIR * p1.x};
+    cx<T> n2 = {-(T)DIR * p2.y, (T)DIR * p2.x};
+    v[0] = {x0.x + a1.x + a2.x, x0.y + a1.y + a2.y};
+    v[1] = {m1.x + n1.x, m1.y + n1.y};
+    v[4] = {m1.x - n1.x, m1.y - n1.y};
+    v[2] = {m2.x + n2.x, m2.y + n2.y};
+    v[3] = {m2.x - n2.x, m2.y - n2.y};
+  }
+}
+
+// One Stockham stage of radix R over nl lines of length K (line stride LS) held in LDS.
+template <typename T, int DIR, int R>
+__device__ __forceinline__ void fft_stage(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst,
+                                          const cx<T>* __restrict__ tw, int K, int LS, int nl, int Ns) {
+  const int KR = K / R;
+  const int items = nl * KR;
+  const int tstep = K / (Ns * R);
+  for (int w = threadIdx.x; w < items; w += blockDim.x) {
+    const int line = w / KR;
+    const int j = w - line * KR;
+    const cx<T>* in = src + line * LS;
+    cx<T>* o = dst + line * LS;
+    const int jm = j % Ns;
+    cx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cx<T> x = in[j + r * KR];
+      if (r > 0) {
+        cx<T> t = tw[jm * r * tstep];
+        T ty = DIR < 0 ? t.y : -t.y;
+        x = {x.x * t.x - x.y * ty, x.x * ty + x.y * t.x};
+      }
+      v[r] = x;
+    }
+    butterfly<T, DIR, R>(v);
+    const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) o[idxD + r * Ns] = v[r];
+  }
+}
+
+// Stockham transform of nl lines of length K (line stride LS) held in LDS.
+// Returns the buffer holding the result (a or b).
+template <typename T, int DIR>
+__device__ cx<T>* fft_lines(cx<T>* a, cx<T>* b, const cx<T>* tw, int K, int LS, int nl, const FftPlan& plan) {
+  cx<T>* src = a;
+  cx<T>* dst = b;
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.radix[s];
+    if (R == 4)
+      fft_stage<T, DIR, 4>(src, dst, tw, K, LS, nl, Ns);
+    else if (R == 2)
+      fft_stage<T, DIR, 2>(src, dst, tw, K, LS, nl, Ns);
+    else if (R == 3)
+      fft_stage<T, DIR, 3>(src, dst, tw, K, LS, nl, Ns);
+    else
+      fft_stage<T, DIR, 5>(src, dst, tw, K, LS, nl, Ns);
+    __syncthreads();
+    cx<T>* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+enum PassMode { PASS_Z_FWD = 0, PASS_Y_FWD = 1, PASS_X_CONV = 2, PASS_Y_INV = 3, PASS_Z_INV = 4 };
+
+struct PassArgs {
+  int A, K, C;   // element (a, k, c) at (a*K + k)*C + c
+  int AB, CB;    // tile: AB values of a, CB values of c
+  int LS;        // LDS line stride
+};
+
+struct ConvArgs {
+  int K[3];
+  double inv[9];
+  double vol;
+  double alpha;
+  int grid_fixed;         // forward z pass input: 0 = float64 grid, 1 = int64 fixed-point grid
+  double grid_inv_scale;  // fixed-point scale of that grid (value = integer * grid_inv_scale)
+};
+
+template <typename T>
+__device__ __forceinline__ T signed_m(int m, int K) {
+  return (T)(m <= K / 2 ? m : m - K);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict__ in_v, void* __restrict__ out_v,
+                                                          PassArgs pa, FftPlan plan, ConvArgs cv,
+                                                          const T* __restrict__ modx, const T* __restrict__ mody,
+                                                          const T* __restrict__ modz, int modz_half,
+                                                          double* __restrict__ e_part) {
+  extern __shared__ unsigned char lds_raw[];
+  const int K = pa.K, LS = pa.LS;
+  const int nl = pa.AB * pa.CB;
+  cx<T>* tw = reinterpret_cast<cx<T>*>(lds_raw);
+  cx<T>* bufA = tw + K;
+  cx<T>* bufB = bufA + nl * LS;
+  // twiddles exp(-2 pi i m / K)
+  for (int m = threadIdx.x; m < K; m += blockDim.x) {
+    double sn, cs;
+    sincospi(-2.0 * (double)m / (double)K, &sn, &cs);
+    tw[m] = {(T)cs, (T)sn};
+  }
+  const int ntc = (pa.C + pa.CB - 1) / pa.CB;
+  const int tile = blockIdx.x;
+  const int a0 = (tile / ntc) * pa.AB;
+  const int c0 = (tile % ntc) * pa.CB;
+  const int nelem = nl * K;
+  // load
+  for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
+    int cb = idx % pa.CB;
+    int rest = idx / pa.CB;
+    int k = rest % K;
+    int ab = rest / K;
+    int a = a0 + ab, c = c0 + cb;
+    cx<T> v = {(T)0, (T)0};
+    if (a < pa.A && c < pa.C) {
+      long long g = ((long long)a * K + k) * pa.C + c;
+      if (MODE == PASS_Z_FWD) {
+        if (cv.grid_fixed)
+          v.x = (T)((double)reinterpret_cast<const long long*>(in_v)[g] * cv.grid_inv_scale);
+        else
+          v.x = (T)reinterpret_cast<const double*>(in_v)[g];
+      } else {
+        v = reinterpret_cast<const cx<T>*>(in_v)[g];
+      }
+    }
+    bufA[(ab * pa.CB + cb) * LS + k] = v;
+  }
+  __syncthreads();
+  cx<T>* res;
+  if (MODE == PASS_Z_FWD || MODE == PASS_Y_FWD || MODE == PASS_X_CONV)
+    res = fft_lines<T, -1>(bufA, bufB, tw, K, LS, nl, plan);
+  else
+    res = fft_lines<T, 1>(bufA, bufB, tw, K, LS, nl, plan);
+  if (MODE == PASS_X_CONV) {
+    // lines are (y, z) = c; along the line k = mx
+    double e_acc = 0.0;
+    const T two_pi = (T)6.283185307179586;
+    for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
+      int line = idx / K;
+      int k = idx - line * K;
+      int c = c0 + line;  // AB == 1 for this pass
+      if (c >= pa.C) continue;
+      int my = c / cv.K[2], mz = c - my * cv.K[2];
+      T fx = signed_m<T>(k, cv.K[0]), fy = signed_m<T>(my, cv.K[1]), fz = signed_m<T>(mz, cv.K[2]);
+      T kx = two_pi * (fx * (T)cv.inv[0] + fy * (T)cv.inv[1] + fz * (T)cv.inv[2]);
+      T ky = two_pi * (fx * (T)cv.inv[3] + fy * (T)cv.inv[4] + fz * (T)cv.inv[5]);
+      T kz = two_pi * (fx * (T)cv.inv[6] + fy * (T)cv.inv[7] + fz * (T)cv.inv[8]);
+      T ksq = kx * kx + ky * ky + kz * kz;
+      cx<T>& s = res[line * LS + k];
+      T G = 0;
+      if (k != 0 || my != 0 || mz != 0) {
+        int zi = modz_half ? (mz <= cv.K[2] / 2 ? mz : cv.K[2] - mz) : mz;
+        T th = modx[k] * mody[my] * modz[zi];
+        G = (T)(4.0 * 3.14159265358979323846 / cv.vol) / ksq * exp(-ksq / (T)(4.0 * cv.alpha * cv.alpha)) / (th * th);
+      }
+      e_acc += 0.5 * (double)G * ((double)s.x * (double)s.x + (double)s.y * (double)s.y);
+      s.x *= G;
+      s.y *= G;
+    }
+    __syncthreads();
+    cx<T>* other = (res == bufA) ? bufB : bufA;
+    res = fft_lines<T, 1>(res, other, tw, K, LS, nl, plan);
+    double tot = tff_block_sum(e_acc);
+    if (threadIdx.x == 0) e_part[blockIdx.x] = tot;
+  }
+  // store
+  for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
+    int cb = idx % pa.CB;
+    int rest = idx / pa.CB;
+    int k = rest % K;
+    int ab = rest / K;
+    int a = a0 + ab, c = c0 + cb;
+    if (a < pa.A && c < pa.C) {
+      long long g = ((long long)a * K + k) * pa.C + c;
+      cx<T> v = res[(ab * pa.CB + cb) * LS + k];
+      if (MODE == PASS_Z_INV)
+        reinterpret_cast<T*>(out_v)[g] = v.x;
+      else
+        reinterpret_cast<cx<T>*>(out_v)[g] = v;
+    }
+  }
+}
+
+struct PassCfg {
+  PassArgs pa;
+  int tiles;
+  size_t lds;
+};
+
+PassCfg pass_cfg(int A, int K, int C, size_t celem) {
+  PassCfg c{};
+  int LS = K + 1;
+  size_t per_line = 2 * (size_t)LS * celem;
+  int lines = (int)((FFT_LDS_BUDGET - (size_t)K * celem) / per_line);
+  if (lines < 1) lines = 1;
+  if (lines > 16) lines = 16;  // more, smaller tiles: >= 512 workgroups for a 96^3 grid
+  c.pa.A = A;
+  c.pa.K = K;
+  c.pa.C = C;
+  c.pa.LS = LS;
+  if (C == 1) {
+    c.pa.AB = lines < A ? lines : A;
+    c.pa.CB = 1;
+  } else {
+    c.pa.AB = 1;
+    c.pa.CB = lines < C ? lines : C;
+  }
+  int ntc = (C + c.pa.CB - 1) / c.pa.CB;
+  int nta = (A + c.pa.AB - 1) / c.pa.AB;
+  c.tiles = ntc * nta;
+  c.lds = (size_t)K * celem + 2 * (size_t)c.pa.AB * c.pa.CB * LS * celem;
+  return c;
+}
+
+PmeGeom make_geom(const TffBox& box, const int K[3], int order, double alpha) {
+  PmeGeom g;
+  for (int a = 0; a < 3; ++a) g.K[a] = K[a];
+  for (int k = 0; k < 9; ++k) g.inv[k] = box.inv[k];
+  g.vol = box.det;
+  g.order = order;
+  g.alpha = alpha;
+  return g;
+}
+
+
+}  // namespace
+}  // namespace tffpme

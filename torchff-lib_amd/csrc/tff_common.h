@@ -133,6 +133,13 @@ __device__ __forceinline__ double tff_block_sum(double v) {
   return r;
 }
 
+// Deterministic sum of n partials by one workgroup (fixed strided order + tree); valid in thread 0.
+__device__ __forceinline__ double tff_sum_partials(const double* __restrict__ part, int n) {
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += part[k];
+  return tff_block_sum(acc);
+}
+
 // Fixed-point (int64) accumulation of forces and other per-atom sums: value * 2^FP_SHIFT.
 #define TFF_FP_SCALE 4294967296.0  // 2^32
 __device__ __forceinline__ unsigned long long tff_to_fixed(double v) {

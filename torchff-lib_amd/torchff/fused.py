@@ -1,0 +1,126 @@
+"""Fused energy + forces of the fixed-charge water composition in one device call.
+
+``FusedNonbondedPME`` evaluates what the reference's TIP3P model composes from
+``NeighborList`` + ``Vdw(LennardJones)`` + ``MultipolarInteraction(rank=0, pairs_excl)``
++ ``PME(rank=0)`` (examples/tip3p/model.py:100-123): Lennard-Jones with
+Lorentz-Berthelot mixing and erfc-screened Coulomb over all pairs within the
+cutoff, the (erfc - 1)/r correction on excluded pairs, and smooth PME, returning
+the energy terms and F = -dE/dr.  No pair list is materialised: atoms are
+sorted into 32-atom spatial clusters and the pair kernel runs over cluster-pair
+tiles (libtorchff_hip.so, ``tff_fused_energy_forces``).
+
+precision="mixed": float32 arithmetic, int64 fixed-point accumulation of forces
+and energies, float64 k-space and self sums.  precision="fp64": float64 throughout.
+"""
+
+import math
+
+import torch
+import torch.nn as nn
+
+from . import _lib as L
+from . import bsplines
+from .nblist import NeighborList
+from .workloads import COULOMB_KJMOL
+
+PAIR_OVERFLOW = 1
+EXCL_OVERFLOW = 2
+
+
+class FusedNonbondedPME(nn.Module):
+    def __init__(self, natoms: int, cutoff: float, ewald_alpha: float, max_hkl, order: int = 5,
+                 coul_constant: float = COULOMB_KJMOL, skin: float = 0.0, exclusions=None,
+                 precision: str = "mixed"):
+        super().__init__()
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        self.natoms = int(natoms)
+        self.cutoff = float(cutoff)
+        self.list_cutoff = float(cutoff) + float(skin)
+        self.alpha = float(ewald_alpha)
+        self.K = tuple(int(k) for k in max_hkl) if isinstance(max_hkl, (tuple, list)) else (int(max_hkl),) * 3
+        self.order = int(order)
+        self.coul = float(coul_constant)
+        self.precision = precision
+        self.compute_dtype = torch.float32 if precision == "mixed" else torch.float64
+        if exclusions is not None:
+            rp, col = NeighborList.convert_pairs_coo_to_csr(torch.as_tensor(exclusions), self.natoms)
+            self.register_buffer("excl_row_ptr", rp)
+            self.register_buffer("excl_col_indices", col)
+        else:
+            self.excl_row_ptr = None
+            self.excl_col_indices = None
+        K0, K1, K2 = self.K
+        self.register_buffer("xmoduli", bsplines.compute_bspline_moduli_1d(K0, order=self.order))
+        self.register_buffer("ymoduli", bsplines.compute_bspline_moduli_1d(K1, order=self.order))
+        self.register_buffer("zmoduli", bsplines.compute_bspline_moduli_z(K2, order=self.order))
+        self._ws = None
+        self._ws_key = None
+        self._mod = None
+        self.status = None
+
+    def _workspace(self, box_host, device):
+        lib = L.load()
+        cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
+        key = (tuple(box_host), device)
+        if self._ws is None or self._ws_key != key:
+            nbytes = lib.tff_fused_workspace_bytes(cd, self.natoms, box_host, self.list_cutoff, *self.K)
+            if nbytes == 0:
+                raise RuntimeError("FusedNonbondedPME needs an orthorhombic periodic box")
+            self._ws = L.workspace(nbytes, device)
+            self._ws_key = key
+        return self._ws
+
+    @torch._dynamo.disable
+    def forward(self, coords, box, charges, sigma, epsilon, check: bool = False):
+        """Returns (energy float64 scalar, forces shaped like coords, terms float64 (4,) =
+        [Lennard-Jones, real-space Coulomb incl. exclusion correction, reciprocal, self])."""
+        L.require_cuda(coords, box, charges, sigma, epsilon)
+        lib = L.load()
+        dev = coords.device
+        io = coords.dtype
+        c = coords.detach().contiguous()
+        if c.shape != (self.natoms, 3):
+            raise RuntimeError(f"coords must be ({self.natoms}, 3)")
+        q, s, e = (t.detach().to(io).contiguous() for t in (charges, sigma, epsilon))
+        bh = L.box_host(box)
+        ws = self._workspace(bh, dev)
+        if self._mod is None or self._mod[0].device != dev or self._mod[0].dtype != self.compute_dtype:
+            self._mod = tuple(m.to(device=dev, dtype=self.compute_dtype).contiguous()
+                              for m in (self.xmoduli, self.ymoduli, self.zmoduli))
+        rp = None if self.excl_row_ptr is None else self.excl_row_ptr.to(dev)
+        col = None if self.excl_col_indices is None else self.excl_col_indices.to(dev)
+        terms = torch.empty(4, dtype=torch.float64, device=dev)
+        forces = torch.empty_like(c)
+        status = torch.empty(1, dtype=torch.int32, device=dev)
+        cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
+        rc = lib.tff_fused_energy_forces(L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms,
+                                         bh, L.ptr(rp), L.ptr(col), self.cutoff, self.list_cutoff, self.alpha,
+                                         self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
+                                         L.ptr(terms), L.ptr(forces), L.ptr(status), L.ptr(ws), ws.numel(),
+                                         L.stream_of(c))
+        L.check(rc, "FusedNonbondedPME")
+        self.status = status
+        if check:
+            self.check()
+        return terms.sum(), forces, terms
+
+    def check(self):
+        """Raise if the last call overflowed an internal capacity (synchronises)."""
+        if self.status is None:
+            return
+        st = int(self.status.item())
+        if st & PAIR_OVERFLOW:
+            raise RuntimeError("FusedNonbondedPME: cluster-pair capacity exceeded")
+        if st & EXCL_OVERFLOW:
+            raise RuntimeError("FusedNonbondedPME: an atom has more than 8 exclusion partners")
+
+
+def for_workload(w, precision="mixed"):
+    """A FusedNonbondedPME configured for a water workload dict (see torchff.workloads)."""
+    return FusedNonbondedPME(w["natoms"], w["cutoff"], w["alpha"], w["K"], order=w["order"],
+                             skin=w.get("skin", 0.0), exclusions=w["exclusions"], precision=precision)
+
+
+def ewald_alpha_from_tolerance(cutoff, tol=5e-4):
+    return math.sqrt(-math.log(2.0 * tol)) / cutoff
