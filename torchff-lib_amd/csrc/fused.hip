@@ -50,6 +50,7 @@ struct Geo {
   int nkeys;
   int ncl_max;
   long long pair_cap;
+  int maxn;  // cluster-pair segment length per i-cluster
   int reach[2];
 };
 
@@ -260,14 +261,14 @@ __global__ __launch_bounds__(FB) void k_bbox(const int* __restrict__ ncl_ptr, co
 }
 
 // ---------------------------------------------------------------- 3. cluster pairs
-// One wave per i-cluster; lanes sweep the clusters of the neighbouring columns.
+// One wave per i-cluster; lanes sweep the clusters of the neighbouring columns in a fixed
+// order and write the accepted j-clusters (cj >= ci) into the i-cluster's own segment.
 __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ ncl_ptr, const int* __restrict__ cl_col,
                                                       const int* __restrict__ col_cl_start,
                                                       const float4* __restrict__ bb_c,
                                                       const float4* __restrict__ bb_h, Geo g, float rlist2,
-                                                      int2* __restrict__ pairs, int* __restrict__ npairs,
+                                                      int* __restrict__ seg, int* __restrict__ seg_count,
                                                       int* __restrict__ status) {
-  __shared__ int2 sbuf[FB / TFF_WAVE][128];
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int w = threadIdx.x / TFF_WAVE;
   const int ci = blockIdx.x * (FB / TFF_WAVE) + w;
@@ -278,7 +279,8 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
   const float4 c0 = bb_c[ci], h0 = bb_h[ci];
   const float Lx = (float)g.L[0], Ly = (float)g.L[1], Lz = (float)g.L[2];
   const float iLx = (float)g.invL[0], iLy = (float)g.invL[1], iLz = (float)g.invL[2];
-  int nbuf = 0;
+  int nseg = 0;
+  int* my = seg + (long long)ci * g.maxn;
   const int nx = 2 * g.reach[0] + 1 >= g.ncol[0] ? g.ncol[0] : 2 * g.reach[0] + 1;
   const int ny = 2 * g.reach[1] + 1 >= g.ncol[1] ? g.ncol[1] : 2 * g.reach[1] + 1;
   const int x0 = nx == g.ncol[0] ? 0 : cx - g.reach[0];
@@ -306,42 +308,74 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
           ok = gx * gx + gy * gy + gz * gz <= rlist2;
         }
         unsigned long long m = __ballot(ok);
-        int cnt = __popcll(m);
-        if (ok) sbuf[w][nbuf + __popcll(m & ((1ull << lane) - 1ull))] = make_int2(ci, cj);
-        nbuf += cnt;
-        if (nbuf > 128 - TFF_WAVE) {
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          int base = 0;
-          if (lane == 0) base = atomicAdd(npairs, nbuf);
-          base = __shfl(base, 0, TFF_WAVE);
-          for (int k = lane; k < nbuf; k += TFF_WAVE) {
-            long long pos = (long long)base + k;
-            if (pos < g.pair_cap) pairs[pos] = sbuf[w][k];
-          }
-          if (lane == 0 && (long long)base + nbuf > g.pair_cap) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          nbuf = 0;
+        if (ok) {
+          int k = nseg + __popcll(m & ((1ull << lane) - 1ull));
+          if (k < g.maxn) my[k] = cj;
         }
+        nseg += __popcll(m);
       }
     }
   }
-  if (nbuf > 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int base = 0;
-    if (lane == 0) base = atomicAdd(npairs, nbuf);
-    base = __shfl(base, 0, TFF_WAVE);
-    for (int k = lane; k < nbuf; k += TFF_WAVE) {
-      long long pos = (long long)base + k;
-      if (pos < g.pair_cap) pairs[pos] = sbuf[w][k];
-    }
-    if (lane == 0 && (long long)base + nbuf > g.pair_cap) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
+  if (lane == 0) {
+    if (nseg > g.maxn) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
+    seg_count[ci] = nseg < g.maxn ? nseg : g.maxn;
   }
+}
+
+// exclusive scan of the per-cluster pair counts (single workgroup), total into *ncp
+__global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ ncl_ptr, const int* __restrict__ cnt,
+                                                      int* __restrict__ start, int* __restrict__ ncp) {
+  __shared__ int s[1024];
+  const int m = *ncl_ptr;
+  int t = threadIdx.x;
+  int per = (m + 1023) / 1024;
+  int b = t * per, e = b + per < m ? b + per : m;
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += cnt[k];
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? s[t - 1] : 0;
+  for (int k = b; k < e; ++k) {
+    start[k] = run;
+    run += cnt[k];
+  }
+  if (t == 1023) *ncp = s[1023];
+}
+
+// segments -> one compact cluster-pair list in (ci, segment order): deterministic
+__global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ ncl_ptr, const int* __restrict__ seg,
+                                                      const int* __restrict__ seg_count,
+                                                      const int* __restrict__ seg_start, int maxn,
+                                                      int2* __restrict__ pairs) {
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const int ci = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
+  if (ci >= *ncl_ptr) return;
+  const int n = seg_count[ci], o = seg_start[ci];
+  for (int k = lane; k < n; k += TFF_WAVE) pairs[o + k] = make_int2(ci, seg[(long long)ci * maxn + k]);
+}
+
+// order the atoms of each sort key by their original index (the scatter used atomics)
+__global__ __launch_bounds__(FB) void k_sort_keys(int nkeys, const int* __restrict__ key_start,
+                                                  int* __restrict__ sorted_orig, int* __restrict__ pos_of) {
+  int k = blockIdx.x * FB + threadIdx.x;
+  if (k >= nkeys) return;
+  const int b = key_start[k], e = key_start[k + 1];
+  for (int p = b + 1; p < e; ++p) {
+    int v = sorted_orig[p];
+    int q = p - 1;
+    while (q >= b && sorted_orig[q] > v) {
+      sorted_orig[q + 1] = sorted_orig[q];
+      --q;
+    }
+    sorted_orig[q + 1] = v;
+  }
+  for (int p = b; p < e; ++p) pos_of[sorted_orig[p]] = p;
 }
 
 // ---------------------------------------------------------------- 4. tile kernel
@@ -679,20 +713,20 @@ bool make_geo(const TffBox& box, long long n, double list_cutoff, Geo& g) {
   g.zbw = g.L[2] / nzb;
   g.nkeys = g.ncol[0] * g.ncol[1] * g.nzb;
   g.ncl_max = (int)((n + CL - 1) / CL) + g.ncol[0] * g.ncol[1];
-  // cluster-pair capacity: every cluster against the clusters of its column neighbourhood, bounded by all pairs
-  long long ncl = g.ncl_max;
-  long long all = ncl * (ncl + 1) / 2;
-  int nx = 2 * g.reach[0] + 1 > g.ncol[0] ? g.ncol[0] : 2 * g.reach[0] + 1;
-  int ny = 2 * g.reach[1] + 1 > g.ncol[1] ? g.ncol[1] : 2 * g.reach[1] + 1;
-  double per_col = (double)ncl / (g.ncol[0] * g.ncol[1]) + 1.0;
-  long long geo = (long long)(ncl * (double)nx * ny * per_col * 0.6) + 1024;
-  g.pair_cap = geo < all ? geo : all;
+  // cluster pairs per i-cluster (half list): clusters whose cube of edge a lies within the
+  // dilated cube of edge a + 2 rlist, with a 2.5x margin for cluster shapes, in multiples of 64
+  double dil = (a + 2.0 * list_cutoff) / a;
+  int maxn = (int)ceil(dil * dil * dil * 1.25 / 64.0) * 64;
+  if (maxn < 128) maxn = 128;
+  if (maxn > g.ncl_max) maxn = ((g.ncl_max + 63) / 64) * 64;
+  g.maxn = maxn;
+  g.pair_cap = (long long)g.ncl_max * maxn;
   return true;
 }
 
 struct FusedWs {
   int *key_of, *key_count, *key_start, *cursor, *sorted_orig, *pos_of, *col_cl_start, *ncl, *cl_col, *slot_orig,
-      *excl_part, *ncp;
+      *excl_part, *ncp, *seg, *seg_count, *seg_start;
   void *xyzq, *lj;
   float4 *bb_c, *bb_h;
   int2* cpairs;
@@ -733,6 +767,9 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   w.bb_c = (float4*)take(16 * (size_t)g.ncl_max);
   w.bb_h = (float4*)take(16 * (size_t)g.ncl_max);
   w.cpairs = (int2*)take(8 * (size_t)g.pair_cap);
+  w.seg = (int*)take(4 * (size_t)g.pair_cap);
+  w.seg_count = (int*)take(4 * (size_t)g.ncl_max);
+  w.seg_start = (int*)take(4 * (size_t)g.ncl_max);
   w.facc = (unsigned long long*)take(8 * 3 * slots);
   w.eacc = (unsigned long long*)take(8 * 4);
   w.grid = (unsigned long long*)take(8 * ng);
@@ -803,6 +840,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.key_start, ncols, g.nzb, w.col_cl_start, w.ncl);
   hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.key_of, w.key_start, w.cursor,
                      w.sorted_orig, w.pos_of);
+  hipLaunchKernelGGL(k_sort_keys, dim3((g.nkeys + FB - 1) / FB), dim3(FB), 0, st, g.nkeys, w.key_start,
+                     w.sorted_orig, w.pos_of);
   // 2. cluster arrays
   zero32(w.slot_orig, nslots);
   hipLaunchKernelGGL((k_fill_clusters<TI, TC>), dim3(nbS), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
@@ -819,7 +858,10 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     void* tok = tff_prof_begin("fused_cluster_pairs", st);
     float rl2 = (float)(list_cutoff * list_cutoff);
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c,
-                       w.bb_h, g, rl2, w.cpairs, w.ncp, status);
+                       w.bb_h, g, rl2, w.seg, w.seg_count, status);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp);
+    hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
+                       w.cpairs);
     tff_prof_end(tok, st);
   }
   // 4. tiles
