@@ -157,6 +157,7 @@ def main():
         f = Fused(w, dev, "mixed")
         sec, out = time_steps(f.step, args.steps, args.warmup)
         f.m.check()
+        tiles = f.m.tiles()
         res["mixed"] = sec
         outs["mixed"] = (float(out[0]), out[1].double().cpu().numpy())
         kern = kernel_times(f.step, args.steps,
@@ -181,7 +182,8 @@ def main():
         achieved = p_in * PAIR_FLOPS / (t * 1e-3) / 1e12
         roof = {"kernel": "fused_tiles", "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": t,
-                "units": f"{p_in} in-cutoff pairs x {PAIR_FLOPS} FLOP per launch"}
+                "units": f"{p_in} in-cutoff pairs x {PAIR_FLOPS} FLOP per launch",
+                "tiles": tiles, "pair_slot_efficiency": p_in / (tiles * 1024.0)}
     out = {
         "metric": "energy+force evals/s, 96k-atom PME water",
         "value": 1.0 / sec,

@@ -131,7 +131,8 @@ size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const doub
  * fixed-point force / energy accumulation and float64 k-space sums, 1 = float64.
  * energies_out (device, 4 doubles): Lennard-Jones, real-space Coulomb (incl. exclusion
  * correction), reciprocal, self.  forces_out (io_dtype, N x 3) = -dE/dr.  status_out
- * (device int32): TFF_FUSED_* flags, 0 when complete.  Moduli as for tff_pme_energy with
+ * (device, 2 int32): [0] TFF_FUSED_* flags, 0 when complete; [1] number of 32 x 32
+ * cluster-pair tiles evaluated.  Moduli as for tff_pme_energy with
  * zmoduli of length K2/2 + 1, in compute_dtype. */
 int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
                             const void* sigma, const void* epsilon, long long natoms, const double* box_host,

@@ -2,7 +2,7 @@
 
 Tolerances: fp64 mode energy terms to 1e-9 of the summed |terms| and force RMS
 1e-9; mixed mode (float32 arithmetic, int64 fixed-point accumulation) energy to
-2e-6 of the summed |terms| and force RMS relative error <= 1e-4 (the stated
+5e-8 of the summed |terms| and force RMS relative error <= 1e-4 (the stated
 target for the 96k-atom box).
 """
 
@@ -37,7 +37,7 @@ def _rms_rel(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
 
 
-@pytest.mark.parametrize("precision,te,tf", [("fp64", 1e-9, 1e-9), ("mixed", 2e-6, 1e-4)])
+@pytest.mark.parametrize("precision,te,tf", [("fp64", 1e-9, 1e-9), ("mixed", 5e-8, 1e-4)])
 def test_fused_matches_oracle(case, precision, te, tf):
     w, E, F, terms = case
     Ed, Fd, td = _run(w, precision)

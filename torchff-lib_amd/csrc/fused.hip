@@ -143,15 +143,6 @@ __global__ __launch_bounds__(FB) void k_scatter_atoms(long long n, const int* __
   pos_of[i] = p;
 }
 
-// slot index (cluster * 32 + s) of sorted position p
-__device__ __forceinline__ int slot_of_pos(int p, int col, const int* key_start, const int* col_cl_start, int nzb) {
-  int cs = key_start[col * nzb];
-  int r = p - cs;
-  return (col_cl_start[col] + r / CL) * CL + (r % CL);
-}
-
-__device__ __forceinline__ int column_of_key(int key, int nzb) { return key / nzb; }
-
 // ---------------------------------------------------------------- 2. cluster arrays
 // One thread per slot of the padded cluster arrays (ncl * 32 slots).
 template <typename TI, typename TC>
@@ -195,17 +186,64 @@ __global__ __launch_bounds__(FB) void k_fill_clusters(const TI* __restrict__ coo
   }
 }
 
+// Reorder the 32 slots of every cluster into four spatially compact groups of 8: sort by x
+// and split in halves, sort each half by y and split, order each group by z (padding slots,
+// keyed +inf, collect at the end; ties broken by slot).  One wave per cluster, lanes 0..31.
+// Also writes slot_of_orig[i], the final slot of atom i.
+template <typename TC>
+__global__ __launch_bounds__(FB) void k_reorder_clusters(const int* __restrict__ ncl_ptr, V4<TC>* __restrict__ xyzq,
+                                                         V2<TC>* __restrict__ lj, int* __restrict__ slot_orig,
+                                                         int* __restrict__ slot_of_orig) {
+  const int c = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  if (c >= *ncl_ptr) return;
+  const int s = lane & (CL - 1);
+  const int sl = c * CL + s;
+  const V4<TC> p = xyzq[sl];
+  const V2<TC> l = lj[sl];
+  const int o = slot_orig[sl];
+  const float inf = 3.0e38f;
+  const float kx = o >= 0 ? (float)p.x : inf, ky = o >= 0 ? (float)p.y : inf, kz = o >= 0 ? (float)p.z : inf;
+  // rank by x over all 32 slots
+  int rx = 0;
+  for (int t = 0; t < CL; ++t) {
+    float vx = __shfl(kx, t, TFF_WAVE);
+    rx += (vx < kx || (vx == kx && t < s)) ? 1 : 0;
+  }
+  const int half = rx >= CL / 2;
+  int ry = 0;
+  for (int t = 0; t < CL; ++t) {
+    float vy = __shfl(ky, t, TFF_WAVE);
+    int ht = __shfl(half, t, TFF_WAVE);
+    ry += (ht == half && (vy < ky || (vy == ky && t < s))) ? 1 : 0;
+  }
+  const int grp = 2 * half + (ry >= CL / 4);
+  int rz = 0;
+  for (int t = 0; t < CL; ++t) {
+    float vz = __shfl(kz, t, TFF_WAVE);
+    int gt = 2 * __shfl(half, t, TFF_WAVE);
+    int rt = __shfl(ry, t, TFF_WAVE);
+    gt += rt >= CL / 4;
+    rz += (gt == grp && (vz < kz || (vz == kz && t < s))) ? 1 : 0;
+  }
+  const int ns = c * CL + grp * 8 + rz;
+  __builtin_amdgcn_wave_barrier();
+  if (lane < CL) {
+    xyzq[ns] = p;
+    lj[ns] = l;
+    slot_orig[ns] = o;
+    if (o >= 0) slot_of_orig[o] = ns;
+  }
+}
+
 // exclusion partners of every slot as packed (cluster << 5 | slot), -1 = none
-__global__ __launch_bounds__(FB) void k_fill_exclusions(long long n, const int* __restrict__ pos_of,
-                                                        const int* __restrict__ key_of, Geo g,
-                                                        const int* __restrict__ key_start,
-                                                        const int* __restrict__ col_cl_start,
+__global__ __launch_bounds__(FB) void k_fill_exclusions(long long n, const int* __restrict__ slot_of_orig,
                                                         const int64_t* __restrict__ row_ptr,
                                                         const int64_t* __restrict__ col,
                                                         int* __restrict__ excl_part, int* __restrict__ status) {
   long long i = blockIdx.x * (long long)FB + threadIdx.x;
   if (i >= n) return;
-  int si = slot_of_pos(pos_of[i], column_of_key(key_of[i], g.nzb), key_start, col_cl_start, g.nzb);
+  int si = slot_of_orig[i];
   int k = 0;
   if (col) {
     for (long long e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
@@ -215,8 +253,7 @@ __global__ __launch_bounds__(FB) void k_fill_exclusions(long long n, const int* 
         atomicOr(status, TFF_FUSED_EXCL_OVERFLOW);
         break;
       }
-      excl_part[si * MAXEX + k++] =
-          slot_of_pos(pos_of[j], column_of_key(key_of[j], g.nzb), key_start, col_cl_start, g.nzb);
+      excl_part[si * MAXEX + k++] = slot_of_orig[j];
     }
   }
   for (; k < MAXEX; ++k) excl_part[si * MAXEX + k] = -1;
@@ -231,32 +268,51 @@ __global__ void k_clear_pad_exclusions(const int* __restrict__ ncl_ptr, const in
     for (int k = 0; k < MAXEX; ++k) excl_part[sl * MAXEX + k] = -1;
 }
 
-// cluster bounding boxes (centre, half extent): one wave per cluster, lanes 0..31
+// cluster bounding boxes (centre, half extent) of the whole cluster and of its four groups of
+// 8 slots; an empty group gets half extent -1.  One wave per cluster, lanes 0..31.
 template <typename TC>
 __global__ __launch_bounds__(FB) void k_bbox(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
                                              const int* __restrict__ slot_orig, float4* __restrict__ bb_c,
-                                             float4* __restrict__ bb_h) {
+                                             float4* __restrict__ bb_h, float4* __restrict__ sb_c,
+                                             float4* __restrict__ sb_h) {
   int c = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
   int lane = threadIdx.x & (TFF_WAVE - 1);
   if (c >= *ncl_ptr) return;
   int sl = c * CL + (lane & (CL - 1));
   bool v = slot_orig[sl] >= 0;
   V4<TC> p = xyzq[sl];
-  float mn[3], mx[3];
+  float mn[3], mx[3], gmn[3], gmx[3];
   float pv[3] = {(float)p.x, (float)p.y, (float)p.z};
   for (int a = 0; a < 3; ++a) {
     mn[a] = v ? pv[a] : 3.0e38f;
     mx[a] = v ? pv[a] : -3.0e38f;
-    for (int off = 16; off > 0; off >>= 1) {
+    for (int off = 4; off > 0; off >>= 1) {
+      mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, TFF_WAVE));
+      mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, TFF_WAVE));
+    }
+    gmn[a] = mn[a];
+    gmx[a] = mx[a];
+    for (int off = 16; off > 4; off >>= 1) {
       mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, TFF_WAVE));
       mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, TFF_WAVE));
     }
   }
+  // widen by a relative epsilon so float rounding of the tests never drops a pair
   if (lane == 0) {
     bb_c[c] = make_float4(0.5f * (mn[0] + mx[0]), 0.5f * (mn[1] + mx[1]), 0.5f * (mn[2] + mx[2]), 0.f);
-    // widen by a relative epsilon so float rounding of the test never drops a pair
     bb_h[c] = make_float4(0.5f * (mx[0] - mn[0]) * 1.0001f + 1e-6f, 0.5f * (mx[1] - mn[1]) * 1.0001f + 1e-6f,
                           0.5f * (mx[2] - mn[2]) * 1.0001f + 1e-6f, 0.f);
+  }
+  if (lane < CL && (lane & 7) == 0) {
+    const int gi = c * 4 + lane / 8;
+    if (gmx[0] < gmn[0]) {
+      sb_c[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
+      sb_h[gi] = make_float4(-1.f, -1.f, -1.f, 0.f);
+    } else {
+      sb_c[gi] = make_float4(0.5f * (gmn[0] + gmx[0]), 0.5f * (gmn[1] + gmx[1]), 0.5f * (gmn[2] + gmx[2]), 0.f);
+      sb_h[gi] = make_float4(0.5f * (gmx[0] - gmn[0]) * 1.0001f + 1e-6f, 0.5f * (gmx[1] - gmn[1]) * 1.0001f + 1e-6f,
+                             0.5f * (gmx[2] - gmn[2]) * 1.0001f + 1e-6f, 0.f);
+    }
   }
 }
 
@@ -267,13 +323,28 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
                                                       const int* __restrict__ col_cl_start,
                                                       const float4* __restrict__ bb_c,
                                                       const float4* __restrict__ bb_h, Geo g, float rlist2,
+                                                      const int* __restrict__ excl_part,
                                                       int* __restrict__ seg, int* __restrict__ seg_count,
                                                       int* __restrict__ status) {
+  __shared__ int s_xp[FB / TFF_WAVE][CL * MAXEX];
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int w = threadIdx.x / TFF_WAVE;
   const int ci = blockIdx.x * (FB / TFF_WAVE) + w;
   const int ncl = *ncl_ptr;
   if (ci >= ncl) return;
+  // clusters (other than ci) holding an exclusion partner of an atom of ci, compacted in LDS
+  int nxp = 0;
+  for (int k = 0; k < MAXEX; ++k) {
+    int v = lane < CL ? excl_part[(ci * CL + lane) * MAXEX + k] : -1;
+    int pc = v >= 0 ? (v >> 5) : -1;
+    bool keep = pc >= 0 && pc != ci;
+    unsigned long long m = __ballot(keep);
+    if (keep) s_xp[w][nxp + __popcll(m & ((1ull << lane) - 1ull))] = pc;
+    nxp += __popcll(m);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int col = cl_col[ci];
   const int cx = col / g.ncol[1], cy = col % g.ncol[1];
   const float4 c0 = bb_c[ci], h0 = bb_h[ci];
@@ -309,8 +380,11 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
         }
         unsigned long long m = __ballot(ok);
         if (ok) {
+          // bit 30 marks tiles that contain excluded pairs
+          int xf = cj == ci ? 1 : 0;
+          for (int t = 0; t < nxp; ++t) xf |= s_xp[w][t] == cj ? 1 : 0;
           int k = nseg + __popcll(m & ((1ull << lane) - 1ull));
-          if (k < g.maxn) my[k] = cj;
+          if (k < g.maxn) my[k] = cj | (xf << 30);
         }
         nseg += __popcll(m);
       }
@@ -324,7 +398,8 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
 
 // exclusive scan of the per-cluster pair counts (single workgroup), total into *ncp
 __global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ ncl_ptr, const int* __restrict__ cnt,
-                                                      int* __restrict__ start, int* __restrict__ ncp) {
+                                                      int* __restrict__ start, int* __restrict__ ncp,
+                                                      int* __restrict__ status) {
   __shared__ int s[1024];
   const int m = *ncl_ptr;
   int t = threadIdx.x;
@@ -345,7 +420,10 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ nc
     start[k] = run;
     run += cnt[k];
   }
-  if (t == 1023) *ncp = s[1023];
+  if (t == 1023) {
+    *ncp = s[1023];
+    status[1] = s[1023];
+  }
 }
 
 // segments -> one compact cluster-pair list in (ci, segment order): deterministic
@@ -379,6 +457,36 @@ __global__ __launch_bounds__(FB) void k_sort_keys(int nkeys, const int* __restri
 }
 
 // ---------------------------------------------------------------- 4. tile kernel
+// erfc(u) and exp(-u^2) for u >= 0.  float: library exp, hardware reciprocal, and
+// erfc(u) exp(u^2) = sum_k c_k t^k, t = 1 / (1 + 0.39 u), a degree-8 polynomial fitted to the
+// scaled complementary error function on [0, 3.6] by iteratively reweighted least squares
+// (relative error 1.3e-9 in exact arithmetic, about 1.5e-6 evaluated in float32, against
+// scipy.special.erfc); double: the correctly rounded library functions.
+__device__ __forceinline__ void erfc_exp(float u, float& ec, float& em) {
+  em = expf(-u * u);
+  const float t = __frcp_rn(1.0f + 0.39f * u);
+  float p = 9.27117256e-02f;
+  p = p * t - 5.29840954e-01f;
+  p = p * t + 1.06552963e+00f;
+  p = p * t - 8.61721388e-01f;
+  p = p * t + 7.81784359e-01f;
+  p = p * t - 4.21555312e-02f;
+  p = p * t + 2.82117988e-01f;
+  p = p * t + 2.10995424e-01f;
+  p = p * t + 5.78745155e-04f;
+  ec = p * em;
+}
+__device__ __forceinline__ void erfc_exp(double u, double& ec, double& em) {
+  em = exp(-u * u);
+  ec = erfc(u);
+}
+// hardware reciprocal square root refined by one Newton step (LJ raises its error to the 6th power)
+__device__ __forceinline__ float rsqrt_t(float x) {
+  const float y = rsqrtf(x);
+  return y * (1.5f - 0.5f * x * y * y);
+}
+__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+
 template <typename TC>
 struct TileArgs {
   TC L[3], invL[3];
@@ -387,111 +495,157 @@ struct TileArgs {
   TC coul;
 };
 
-template <typename TC>
-__device__ __forceinline__ void flush_i(int ci, const TC* f, int lane, const int* slot_orig,
-                                        unsigned long long* facc) {
-  // lanes 0..31 hold the combined i-slot forces
-  if (lane < CL) {
-    int sl = ci * CL + lane;
-    if (slot_orig[sl] >= 0) {
-      atomicAdd(&facc[3 * sl + 0], (unsigned long long)to_fx((double)f[0], F_SCALE));
-      atomicAdd(&facc[3 * sl + 1], (unsigned long long)to_fx((double)f[1], F_SCALE));
-      atomicAdd(&facc[3 * sl + 2], (unsigned long long)to_fx((double)f[2], F_SCALE));
-    }
-  }
-}
-
+// One wave per 32 x 32 cluster tile at a time, evaluated as up to 16 sub-tiles of 8 x 8 atoms
+// (the clusters' compact groups of 8): lane = (ii = lane & 7, jj = lane >> 3) holds one pair of
+// a sub-tile, and only sub-tiles whose group bounding boxes are within the list cutoff are
+// evaluated (a wave-uniform loop).  F_i stays in registers across consecutive tiles of the same
+// i-cluster; F_j is reduced over the 8 lanes sharing jj and added with int64 atomics per tile.
 template <typename TC>
 __global__ __launch_bounds__(TILE_BLOCK) void k_tiles(const int2* __restrict__ cpairs, const int* __restrict__ ncp_ptr,
                                                       long long cap, const V4<TC>* __restrict__ xyzq,
                                                       const V2<TC>* __restrict__ lj, const int* __restrict__ slot_orig,
-                                                      const int* __restrict__ excl_part, TileArgs<TC> ta,
+                                                      const int* __restrict__ excl_part, const float4* __restrict__ sb_c,
+                                                      const float4* __restrict__ sb_h, TileArgs<TC> ta, float rlist2,
                                                       unsigned long long* __restrict__ facc,
                                                       unsigned long long* __restrict__ eacc) {
-  __shared__ V4<TC> s_xyzq[TILE_WAVES][CL];
-  __shared__ V2<TC> s_lj[TILE_WAVES][CL];
-  __shared__ TC s_fj[TILE_WAVES][2][3][CL];
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int w = threadIdx.x / TFF_WAVE;
-  const int is = lane & (CL - 1);
-  const int half = lane >> 5;
+  const int ii = lane & 7, jj = lane >> 3;
   long long ncp = *ncp_ptr;
   if (ncp > cap) ncp = cap;
   const long long nw = (long long)gridDim.x * TILE_WAVES;
   const long long gw = (long long)blockIdx.x * TILE_WAVES + w;
-  // contiguous chunk of tiles per wave so consecutive tiles share the i-cluster
   const long long per = (ncp + nw - 1) / nw;
   const long long t0 = gw * per;
   long long t1 = t0 + per;
   if (t1 > ncp) t1 = ncp;
   const TC two_over_sqrt_pi = (TC)1.1283791670955126;
+  const float fL[3] = {(float)ta.L[0], (float)ta.L[1], (float)ta.L[2]};
+  const float fiL[3] = {(float)ta.invL[0], (float)ta.invL[1], (float)ta.invL[2]};
   TC e_lj = 0, e_c = 0;
   int cur_i = -1;
-  TC fi[3] = {0, 0, 0};
-  V4<TC> pi = {0, 0, 0, 0};
-  V2<TC> li = {0, 0};
-  bool vi = false;
-  int ex[MAXEX];
+  TC fi[4][3];
+  V4<TC> pi[4];
+  V2<TC> li[4];
+  bool vi[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    fi[g][0] = fi[g][1] = fi[g][2] = 0;
+    vi[g] = false;
+  }
+  auto flush_i = [&](int c) {
+    // reduce over the 8 lanes sharing ii (lane bits 3..5), lanes 0..7 write
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        TC v = fi[g][a];
+        v += __shfl_xor(v, 8, TFF_WAVE);
+        v += __shfl_xor(v, 16, TFF_WAVE);
+        v += __shfl_xor(v, 32, TFF_WAVE);
+        fi[g][a] = v;
+      }
+    if (jj == 0) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int sl = c * CL + g * 8 + ii;
+        if (vi[g]) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            if (fi[g][a] != (TC)0) atomicAdd(&facc[3 * sl + a], (unsigned long long)to_fx((double)fi[g][a], F_SCALE));
+        }
+      }
+    }
+  };
   for (long long t = t0; t < t1; ++t) {
     const int2 cp = cpairs[t];
-    const int ci = cp.x, cj = cp.y;
+    const int ci = cp.x, cj = cp.y & 0x3fffffff;
+    const bool xflag = (cp.y >> 30) & 1;
     if (ci != cur_i) {
-      if (cur_i >= 0) {
-        // combine the two halves of each i-slot and flush
-        for (int a = 0; a < 3; ++a) fi[a] += __shfl_xor(fi[a], 32, TFF_WAVE);
-        flush_i<TC>(cur_i, fi, lane, slot_orig, facc);
-      }
+      if (cur_i >= 0) flush_i(cur_i);
       cur_i = ci;
-      fi[0] = fi[1] = fi[2] = 0;
-      const int sl = ci * CL + is;
-      pi = xyzq[sl];
-      li = lj[sl];
-      vi = slot_orig[sl] >= 0;
 #pragma unroll
-      for (int k = 0; k < MAXEX; ++k) ex[k] = excl_part[sl * MAXEX + k];
+      for (int g = 0; g < 4; ++g) {
+        const int sl = ci * CL + g * 8 + ii;
+        pi[g] = xyzq[sl];
+        li[g] = lj[sl];
+        vi[g] = slot_orig[sl] >= 0;
+        fi[g][0] = fi[g][1] = fi[g][2] = 0;
+      }
     }
-    // stage the j cluster and clear the j accumulators
-    if (lane < CL) {
-      s_xyzq[w][lane] = xyzq[cj * CL + lane];
-      V2<TC> l = lj[cj * CL + lane];
-      if (slot_orig[cj * CL + lane] < 0) l.y = (TC)-1;  // marks padding
-      s_lj[w][lane] = l;
-    }
-    for (int a = 0; a < 3; ++a) s_fj[w][half][a][is] = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    unsigned int exmask = 0;
-#pragma unroll
-    for (int k = 0; k < MAXEX; ++k)
-      if (ex[k] >= 0 && (ex[k] >> 5) == cj) exmask |= 1u << (ex[k] & 31);
     const bool diag = ci == cj;
-#pragma unroll 4
-    for (int k = 0; k < CL / 2; ++k) {
-      const int js = (is + 16 * half + k) & (CL - 1);
-      const V4<TC> pj = s_xyzq[w][js];
-      const V2<TC> lJ = s_lj[w][js];
-      TC dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    // active sub-tiles: group bounding boxes within the list cutoff (lanes 0..15 test one each)
+    bool sub_ok = false;
+    if (lane < 16) {
+      const int gi = lane >> 2, gj = lane & 3;
+      const float4 c0 = sb_c[ci * 4 + gi], h0 = sb_h[ci * 4 + gi];
+      const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
+      if (h0.x >= 0.f && h1.x >= 0.f && (!diag || gj >= gi)) {
+        float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
+        dx -= fL[0] * rintf(dx * fiL[0]);
+        dy -= fL[1] * rintf(dy * fiL[1]);
+        dz -= fL[2] * rintf(dz * fiL[2]);
+        const float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
+        const float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
+        const float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
+        sub_ok = gx * gx + gy * gy + gz * gz <= rlist2;
+      }
+    }
+    const unsigned int mask = (unsigned int)(__ballot(sub_ok) & 0xffffull);
+    if (mask == 0) continue;
+    // exclusion bit masks of this lane's four i-slots against cj (only for flagged tiles)
+    unsigned int exm[4] = {0u, 0u, 0u, 0u};
+    if (xflag) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int sl = ci * CL + g * 8 + ii;
+#pragma unroll
+        for (int k = 0; k < MAXEX; ++k) {
+          const int pv = excl_part[sl * MAXEX + k];
+          if (pv >= 0 && (pv >> 5) == cj) exm[g] |= 1u << (pv & 31);
+        }
+      }
+    }
+    V4<TC> pj[4];
+    V2<TC> lJ[4];
+    bool vj[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int sl = cj * CL + g * 8 + jj;
+      pj[g] = xyzq[sl];
+      lJ[g] = lj[sl];
+      vj[g] = slot_orig[sl] >= 0;
+    }
+    TC fj[4][3];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) fj[g][0] = fj[g][1] = fj[g][2] = 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      if (!((mask >> b) & 1u)) continue;
+      const int gi = b >> 2, gj = b & 3;
+      TC dx = pi[gi].x - pj[gj].x, dy = pi[gi].y - pj[gj].y, dz = pi[gi].z - pj[gj].z;
       dx -= ta.L[0] * rint(dx * ta.invL[0]);
       dy -= ta.L[1] * rint(dy * ta.invL[1]);
       dz -= ta.L[2] * rint(dz * ta.invL[2]);
       const TC r2 = dx * dx + dy * dy + dz * dz;
-      bool ok = vi && lJ.y >= (TC)0 && r2 <= ta.rc2 && (!diag || js > is);
+      const bool ok = vi[gi] && vj[gj] && r2 <= ta.rc2 && (!(diag && gi == gj) || jj > ii);
       TC fs = 0;
       if (ok) {
-        const bool excl = (exmask >> js) & 1u;
-        const TC rinv = (TC)1 / sqrt(r2);
+        const bool excl = (exm[gi] >> (gj * 8 + jj)) & 1u;
+        const TC rinv = rsqrt_t(r2);
         const TC rinv2 = rinv * rinv;
         const TC r = r2 * rinv;
         const TC u = ta.alpha * r;
-        const TC ec = erfc(u) - (excl ? (TC)1 : (TC)0);
-        const TC ex2 = exp(-u * u) * two_over_sqrt_pi * u;
-        const TC qq = ta.coul * pi.w * pj.w;
+        TC ec, em;
+        erfc_exp(u, ec, em);
+        ec -= excl ? (TC)1 : (TC)0;
+        const TC ex2 = em * two_over_sqrt_pi * u;
+        const TC qq = ta.coul * pi[gi].w * pj[gj].w;
         e_c += qq * ec * rinv;
         fs = qq * (ec + ex2) * rinv * rinv2;
         if (!excl) {
-          const TC sg = li.x + lJ.x;
-          const TC ep = li.y * lJ.y;
+          const TC sg = li[gi].x + lJ[gj].x;
+          const TC ep = li[gi].y * lJ[gj].y;
           const TC s2 = sg * sg * rinv2;
           const TC s6 = s2 * s2 * s2;
           e_lj += (TC)4 * ep * s6 * (s6 - (TC)1);
@@ -499,36 +653,37 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tiles(const int2* __restrict__ c
         }
       }
       // F_i += fs d, F_j -= fs d   (fs = -dE/dr / r)
-      fi[0] += fs * dx;
-      fi[1] += fs * dy;
-      fi[2] += fs * dz;
-      s_fj[w][half][0][js] -= fs * dx;
-      s_fj[w][half][1][js] -= fs * dy;
-      s_fj[w][half][2][js] -= fs * dz;
-      // the next step's lane for this j-slot is another lane: keep the LDS read-modify-writes of
-      // consecutive steps in program order (a wave's LDS operations execute in issue order)
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      fi[gi][0] += fs * dx;
+      fi[gi][1] += fs * dy;
+      fi[gi][2] += fs * dz;
+      fj[gj][0] -= fs * dx;
+      fj[gj][1] -= fs * dy;
+      fj[gj][2] -= fs * dz;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < CL) {
-      const int sl = cj * CL + lane;
-      if (slot_orig[sl] >= 0) {
-        for (int a = 0; a < 3; ++a) {
-          TC v = s_fj[w][0][a][lane] + s_fj[w][1][a][lane];
-          if (v != (TC)0) atomicAdd(&facc[3 * sl + a], (unsigned long long)to_fx((double)v, F_SCALE));
+    // reduce F_j over the 8 lanes sharing jj (lane bits 0..2); lanes with ii == 0 write
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        TC v = fj[g][a];
+        v += __shfl_xor(v, 1, TFF_WAVE);
+        v += __shfl_xor(v, 2, TFF_WAVE);
+        v += __shfl_xor(v, 4, TFF_WAVE);
+        fj[g][a] = v;
+      }
+    if (ii == 0) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int sl = cj * CL + g * 8 + jj;
+        if (vj[g]) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            if (fj[g][a] != (TC)0) atomicAdd(&facc[3 * sl + a], (unsigned long long)to_fx((double)fj[g][a], F_SCALE));
         }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  if (cur_i >= 0) {
-    for (int a = 0; a < 3; ++a) fi[a] += __shfl_xor(fi[a], 32, TFF_WAVE);
-    flush_i<TC>(cur_i, fi, lane, slot_orig, facc);
-  }
+  if (cur_i >= 0) flush_i(cur_i);
   // energies: wave sums in double, then fixed point
   double el = tff_wave_sum((double)e_lj), ecl = tff_wave_sum((double)e_c);
   if (lane == 0 && t1 > t0) {
@@ -538,43 +693,146 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tiles(const int2* __restrict__ c
 }
 
 // ---------------------------------------------------------------- 5. PME on cluster slots
+// Owner-computes spreading: one workgroup per BR^3 block ("brick") of grid points.  The
+// workgroup lists the clusters whose stencils can reach its brick (column neighbourhood,
+// circular interval tests in grid units), accumulates their charges into the brick in LDS
+// with int64 fixed-point atomics (order independent), then stores the brick: every grid
+// point is written once, so the grid needs neither zeroing nor global atomics.
+constexpr int BR = 8;
+constexpr int BRICK_MAXC = 1024;
+
+__device__ __forceinline__ bool circ_overlap(int lo, int len, int b0, int b, int K) {
+  if (len >= K) return true;
+  int d1 = (b0 - lo) % K;
+  if (d1 < 0) d1 += K;
+  int d2 = (lo - b0) % K;
+  if (d2 < 0) d2 += K;
+  return d1 < len || d2 < b;
+}
+
+// grid-index interval [lo, lo + len) touched by the stencils of atoms with scaled coordinate
+// in [slo, shi] (spline order P, anchor of the spline weights in tffpme::anchor_axis)
+__device__ __forceinline__ void stencil_interval(float slo, float shi, int P, int& lo, int& len) {
+  lo = (int)floorf(slo) - P / 2 - 1;
+  int hi = (int)ceilf(shi) + P - P / 2;
+  len = hi - lo + 1;
+}
+
 template <typename TC, int P>
-__global__ __launch_bounds__(FB) void k_spread_slots(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
-                                                     const int* __restrict__ slot_orig, PmeGeom g,
-                                                     unsigned long long* __restrict__ grid) {
-  int sl = blockIdx.x * FB + threadIdx.x;
-  if (sl >= *ncl_ptr * CL || slot_orig[sl] < 0) return;
-  V4<TC> p = xyzq[sl];
-  TC Mv[3][P], dM[3][P];
-  int base[3];
-  const double r[3] = {(double)p.x, (double)p.y, (double)p.z};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double s = (r[0] * g.inv[a] + r[1] * g.inv[3 + a] + r[2] * g.inv[6 + a]) * (double)g.K[a];
-    double t;
-    anchor_axis(s, g.K[a], P, base[a], t);
-    spline_weights<TC, P>((TC)t, Mv[a], dM[a]);
+__global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
+                                                       const int* __restrict__ slot_orig, const float4* __restrict__ bb_c,
+                                                       const float4* __restrict__ bb_h,
+                                                       const int* __restrict__ col_cl_start, Geo g, PmeGeom pg,
+                                                       unsigned long long* __restrict__ grid, int* __restrict__ status) {
+  __shared__ unsigned long long s_q[BR * BR * BR];
+  __shared__ int s_list[BRICK_MAXC];
+  __shared__ int s_n;
+  const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
+  const int nbx = (K0 + BR - 1) / BR, nby = (K1 + BR - 1) / BR, nbz = (K2 + BR - 1) / BR;
+  const int bid = blockIdx.x;
+  if (bid >= nbx * nby * nbz) return;
+  const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
+  const int B0[3] = {bx * BR, by * BR, bz * BR};
+  const int Bn[3] = {K0 - B0[0] < BR ? K0 - B0[0] : BR, K1 - B0[1] < BR ? K1 - B0[1] : BR,
+                     K2 - B0[2] < BR ? K2 - B0[2] : BR};
+  for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) s_q[k] = 0ull;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  // candidate clusters: columns whose x-y extent can reach the brick
+  const float kl[3] = {(float)(pg.K[0] * g.invL[0]), (float)(pg.K[1] * g.invL[1]), (float)(pg.K[2] * g.invL[2])};
+  int c0[2], nc[2];
+  for (int a = 0; a < 2; ++a) {
+    double lo = (B0[a] - P / 2 - 2) * g.L[a] / pg.K[a];
+    double hi = (B0[a] + Bn[a] + P / 2 + 2) * g.L[a] / pg.K[a];
+    int cl = (int)floor(lo / g.colw[a]), ch = (int)floor(hi / g.colw[a]);
+    c0[a] = cl;
+    nc[a] = ch - cl + 1 < g.ncol[a] ? ch - cl + 1 : g.ncol[a];
   }
-  const TC qi = p.w;
-  const int K1 = g.K[1], K2 = g.K[2];
-  int gx = base[0];
-#pragma unroll 1
-  for (int a = 0; a < P; ++a) {
-    int gy = base[1];
-    const TC wa = qi * Mv[0][a];
-#pragma unroll
-    for (int b = 0; b < P; ++b) {
-      const TC wab = wa * Mv[1][b];
-      unsigned long long* row = grid + ((long long)gx * K1 + gy) * K2;
-      int gz = base[2];
-#pragma unroll
-      for (int c = 0; c < P; ++c) {
-        atomicAdd(&row[gz], (unsigned long long)to_fx((double)(wab * Mv[2][c]), Q_SCALE));
-        gz = gz + 1 == K2 ? 0 : gz + 1;
+  const int ncand_cols = nc[0] * nc[1];
+  for (int cc = 0; cc < ncand_cols; ++cc) {
+    int xx = (c0[0] + cc / nc[1]) % g.ncol[0];
+    if (xx < 0) xx += g.ncol[0];
+    int yy = (c0[1] + cc % nc[1]) % g.ncol[1];
+    if (yy < 0) yy += g.ncol[1];
+    const int col = xx * g.ncol[1] + yy;
+    const int cs = col_cl_start[col], ce = col_cl_start[col + 1];
+    for (int c = cs + (int)threadIdx.x; c < ce; c += blockDim.x) {
+      const float4 cc4 = bb_c[c], hh4 = bb_h[c];
+      const float cv[3] = {cc4.x, cc4.y, cc4.z}, hv[3] = {hh4.x, hh4.y, hh4.z};
+      bool ok = true;
+      for (int a = 0; a < 3; ++a) {
+        int lo, len;
+        stencil_interval((cv[a] - hv[a]) * kl[a], (cv[a] + hv[a]) * kl[a], P, lo, len);
+        ok = ok && circ_overlap(lo, len, B0[a], Bn[a], pg.K[a]);
       }
-      gy = gy + 1 == K1 ? 0 : gy + 1;
+      if (ok) {
+        int k = atomicAdd(&s_n, 1);
+        if (k < BRICK_MAXC) s_list[k] = c;
+      }
     }
-    gx = gx + 1 == g.K[0] ? 0 : gx + 1;
+  }
+  __syncthreads();
+  int ncand = s_n;
+  if (ncand > BRICK_MAXC) {
+    if (threadIdx.x == 0) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
+    ncand = BRICK_MAXC;
+  }
+  // two clusters per wave per round: lane >> 5 picks the cluster, lane & 31 the slot
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const int gw = threadIdx.x / TFF_WAVE, nwv = blockDim.x / TFF_WAVE;
+  for (int r = 2 * gw + (lane >> 5); r < ncand; r += 2 * nwv) {
+    const int sl = s_list[r] * CL + (lane & (CL - 1));
+    if (slot_orig[sl] < 0) continue;
+    const V4<TC> p = xyzq[sl];
+    TC Mv[3][P], dM[3][P];
+    int base[3];
+    unsigned int inmask[3];
+    const double rr[3] = {(double)p.x, (double)p.y, (double)p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double sa = (rr[0] * pg.inv[a] + rr[1] * pg.inv[3 + a] + rr[2] * pg.inv[6 + a]) * (double)pg.K[a];
+      double t;
+      anchor_axis(sa, pg.K[a], P, base[a], t);
+      spline_weights<TC, P>((TC)t, Mv[a], dM[a]);
+      unsigned int m = 0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        int gk = base[a] + k;
+        gk = gk >= pg.K[a] ? gk - pg.K[a] : gk;
+        if (gk >= B0[a] && gk < B0[a] + Bn[a]) m |= 1u << k;
+      }
+      inmask[a] = m;
+    }
+    if (!inmask[0] || !inmask[1] || !inmask[2]) continue;
+    const TC qi = p.w;
+#pragma unroll
+    for (int ka = 0; ka < P; ++ka) {
+      if (!((inmask[0] >> ka) & 1u)) continue;
+      int gx = base[0] + ka;
+      gx = gx >= K0 ? gx - K0 : gx;
+      const TC wa = qi * Mv[0][ka];
+#pragma unroll
+      for (int kb = 0; kb < P; ++kb) {
+        if (!((inmask[1] >> kb) & 1u)) continue;
+        int gy = base[1] + kb;
+        gy = gy >= K1 ? gy - K1 : gy;
+        const TC wab = wa * Mv[1][kb];
+        unsigned long long* row = s_q + ((gx - B0[0]) * BR + (gy - B0[1])) * BR;
+#pragma unroll
+        for (int kc = 0; kc < P; ++kc) {
+          if (!((inmask[2] >> kc) & 1u)) continue;
+          int gz = base[2] + kc;
+          gz = gz >= K2 ? gz - K2 : gz;
+          atomicAdd(&row[gz - B0[2]], (unsigned long long)to_fx((double)(wab * Mv[2][kc]), Q_SCALE));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) {
+    const int lz = k % BR, ly = (k / BR) % BR, lx = k / (BR * BR);
+    if (lx < Bn[0] && ly < Bn[1] && lz < Bn[2])
+      grid[((long long)(B0[0] + lx) * K1 + (B0[1] + ly)) * K2 + (B0[2] + lz)] = s_q[k];
   }
 }
 
@@ -726,9 +984,9 @@ bool make_geo(const TffBox& box, long long n, double list_cutoff, Geo& g) {
 
 struct FusedWs {
   int *key_of, *key_count, *key_start, *cursor, *sorted_orig, *pos_of, *col_cl_start, *ncl, *cl_col, *slot_orig,
-      *excl_part, *ncp, *seg, *seg_count, *seg_start;
+      *excl_part, *ncp, *seg, *seg_count, *seg_start, *slot_of_orig;
   void *xyzq, *lj;
-  float4 *bb_c, *bb_h;
+  float4 *bb_c, *bb_h, *sb_c, *sb_h;
   int2* cpairs;
   unsigned long long *facc, *eacc, *grid;
   void* work;
@@ -766,6 +1024,9 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   w.lj = take(2 * tc * slots);
   w.bb_c = (float4*)take(16 * (size_t)g.ncl_max);
   w.bb_h = (float4*)take(16 * (size_t)g.ncl_max);
+  w.sb_c = (float4*)take(16 * 4 * (size_t)g.ncl_max);
+  w.sb_h = (float4*)take(16 * 4 * (size_t)g.ncl_max);
+  w.slot_of_orig = (int*)take(4 * nn);
   w.cpairs = (int2*)take(8 * (size_t)g.pair_cap);
   w.seg = (int*)take(4 * (size_t)g.pair_cap);
   w.seg_count = (int*)take(4 * (size_t)g.ncl_max);
@@ -783,13 +1044,14 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
 }
 
 template <typename TC>
-int launch_pme_slots(int order, const FusedWs& w, const PmeGeom& pg, int nslots, const TC* phi, double coul,
-                     bool spread, hipStream_t st) {
+int launch_pme_slots(int order, const FusedWs& w, const Geo& g, const PmeGeom& pg, int nslots, const TC* phi,
+                     double coul, bool spread, int* status, hipStream_t st) {
   unsigned nb = (unsigned)((nslots + FB - 1) / FB);
+  const unsigned nbricks = (unsigned)(((pg.K[0] + BR - 1) / BR) * ((pg.K[1] + BR - 1) / BR) * ((pg.K[2] + BR - 1) / BR));
 #define TFF_SP(P)                                                                                               \
   if (spread)                                                                                                   \
-    hipLaunchKernelGGL((k_spread_slots<TC, P>), dim3(nb), dim3(FB), 0, st, w.ncl, (const V4<TC>*)w.xyzq,        \
-                       w.slot_orig, pg, w.grid);                                                                 \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P>), dim3(nbricks), dim3(256), 0, st, w.ncl, (const V4<TC>*)w.xyzq, \
+                       w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status);                      \
   else                                                                                                          \
     hipLaunchKernelGGL((k_gather_slots<TC, P>), dim3(nb), dim3(FB), 0, st, w.ncl, phi, (const V4<TC>*)w.xyzq,   \
                        w.slot_orig, pg, coul, w.facc)
@@ -830,10 +1092,9 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   zero32(w.key_count, g.nkeys + 1);
   zero32(w.cursor, g.nkeys + 1);
   zero32(w.ncp, 4);
-  zero32(status, 1);
+  zero32(status, 2);
   zero64(w.facc, 3LL * nslots);
   zero64(w.eacc, 4);
-  zero64(w.grid, (long long)K[0] * K[1] * K[2]);
   // 1. sort
   hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, n, g, w.key_of, w.key_count);
   hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, w.key_count, g.nkeys, w.key_start);
@@ -847,19 +1108,21 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   hipLaunchKernelGGL((k_fill_clusters<TI, TC>), dim3(nbS), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
                      w.col_cl_start, w.key_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
                      w.cl_col);
-  hipLaunchKernelGGL(k_clear_pad_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.excl_part);
-  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbA), dim3(FB), 0, st, n, w.pos_of, w.key_of, g, w.key_start,
-                     w.col_cl_start, row_ptr, col, w.excl_part, status);
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
+  hipLaunchKernelGGL(k_reorder_clusters<TC>, dim3(nbW), dim3(FB), 0, st, w.ncl, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj,
+                     w.slot_orig, w.slot_of_orig);
+  hipLaunchKernelGGL(k_clear_pad_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.excl_part);
+  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbA), dim3(FB), 0, st, n, w.slot_of_orig, row_ptr, col, w.excl_part,
+                     status);
   hipLaunchKernelGGL(k_bbox<TC>, dim3(nbW), dim3(FB), 0, st, w.ncl, (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c,
-                     w.bb_h);
+                     w.bb_h, w.sb_c, w.sb_h);
   // 3. cluster pairs
   {
     void* tok = tff_prof_begin("fused_cluster_pairs", st);
     float rl2 = (float)(list_cutoff * list_cutoff);
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c,
-                       w.bb_h, g, rl2, w.seg, w.seg_count, status);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp);
+                       w.bb_h, g, rl2, w.excl_part, w.seg, w.seg_count, status);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
     hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
                        w.cpairs);
     tff_prof_end(tok, st);
@@ -879,7 +1142,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     if (blocks < 1) blocks = 1;
     void* tok = tff_prof_begin("fused_tiles", st);
     hipLaunchKernelGGL(k_tiles<TC>, dim3((unsigned)blocks), dim3(TILE_BLOCK), 0, st, w.cpairs, w.ncp, g.pair_cap,
-                       (const V4<TC>*)w.xyzq, (const V2<TC>*)w.lj, w.slot_orig, w.excl_part, ta, w.facc, w.eacc);
+                       (const V4<TC>*)w.xyzq, (const V2<TC>*)w.lj, w.slot_orig, w.excl_part, w.sb_c, w.sb_h, ta,
+                       (float)(list_cutoff * list_cutoff), w.facc, w.eacc);
     tff_prof_end(tok, st);
   }
   // 5. PME
@@ -888,7 +1152,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
   {
     void* tok = tff_prof_begin("fused_spread", st);
-    int rc = launch_pme_slots<TC>(order, w, pg, nslots, nullptr, coul, true, st);
+    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, nullptr, coul, true, status, st);
     tff_prof_end(tok, st);
     if (rc) return rc;
   }
@@ -920,7 +1184,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   }
   {
     void* tok = tff_prof_begin("fused_gather", st);
-    int rc = launch_pme_slots<TC>(order, w, pg, nslots, phi, coul, false, st);
+    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, phi, coul, false, status, st);
     tff_prof_end(tok, st);
     if (rc) return rc;
   }

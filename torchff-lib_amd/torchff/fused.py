@@ -88,11 +88,14 @@ class FusedNonbondedPME(nn.Module):
         if self._mod is None or self._mod[0].device != dev or self._mod[0].dtype != self.compute_dtype:
             self._mod = tuple(m.to(device=dev, dtype=self.compute_dtype).contiguous()
                               for m in (self.xmoduli, self.ymoduli, self.zmoduli))
-        rp = None if self.excl_row_ptr is None else self.excl_row_ptr.to(dev)
-        col = None if self.excl_col_indices is None else self.excl_col_indices.to(dev)
+        if self.excl_row_ptr is not None and self.excl_row_ptr.device != dev:
+            # keep the exclusion CSR resident on the device (moved once, not per call)
+            self.excl_row_ptr = self.excl_row_ptr.to(dev)
+            self.excl_col_indices = self.excl_col_indices.to(dev)
+        rp, col = self.excl_row_ptr, self.excl_col_indices
         terms = torch.empty(4, dtype=torch.float64, device=dev)
         forces = torch.empty_like(c)
-        status = torch.empty(1, dtype=torch.int32, device=dev)
+        status = torch.empty(2, dtype=torch.int32, device=dev)
         cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
         rc = lib.tff_fused_energy_forces(L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms,
                                          bh, L.ptr(rp), L.ptr(col), self.cutoff, self.list_cutoff, self.alpha,
@@ -105,11 +108,15 @@ class FusedNonbondedPME(nn.Module):
             self.check()
         return terms.sum(), forces, terms
 
+    def tiles(self):
+        """Number of 32 x 32 cluster-pair tiles of the last call (synchronises)."""
+        return None if self.status is None else int(self.status[1].item())
+
     def check(self):
         """Raise if the last call overflowed an internal capacity (synchronises)."""
         if self.status is None:
             return
-        st = int(self.status.item())
+        st = int(self.status[0].item())
         if st & PAIR_OVERFLOW:
             raise RuntimeError("FusedNonbondedPME: cluster-pair capacity exceeded")
         if st & EXCL_OVERFLOW:
