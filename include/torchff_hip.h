@@ -118,6 +118,8 @@ int tff_pme_gather(int dtype, const void* phi_grid, const void* coords, const vo
 /* ------------------------------------------------- fused energy + forces */
 #define TFF_FUSED_PAIR_OVERFLOW 1  /* cluster-pair capacity exceeded: results incomplete */
 #define TFF_FUSED_EXCL_OVERFLOW 2  /* an atom has more than 8 exclusion partners */
+#define TFF_FUSED_COLUMN_OVERFLOW 4 /* an x-y column holds more than 8192 atoms (left unsorted:
+                                       results valid but not bitwise reproducible) */
 
 size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const double* box_host, double list_cutoff,
                                  int K0, int K1, int K2);
@@ -129,11 +131,12 @@ size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const doub
  * examples/tip3p/model.py:100-123, without materialising a pair list.  Orthorhombic box,
  * each edge at least 2 * list_cutoff.  compute_dtype 0 = float32 arithmetic with int64
  * fixed-point force / energy accumulation and float64 k-space sums, 1 = float64.
- * energies_out (device, 4 doubles): Lennard-Jones, real-space Coulomb (incl. exclusion
- * correction), reciprocal, self.  forces_out (io_dtype, N x 3) = -dE/dr.  status_out
+ * energies_out (device, 5 doubles): Lennard-Jones, real-space Coulomb (incl. exclusion
+ * correction), reciprocal, self, total.  forces_out (io_dtype, N x 3) = -dE/dr.  status_out
  * (device, 2 int32): [0] TFF_FUSED_* flags, 0 when complete; [1] number of 32 x 32
- * cluster-pair tiles evaluated.  Moduli as for tff_pme_energy with
- * zmoduli of length K2/2 + 1, in compute_dtype. */
+ * cluster-pair tiles within the list cutoff (their 8 x 8 sub-tiles are pruned at the
+ * cutoff on every call).  Moduli as for tff_pme_energy with zmoduli of length K2/2 + 1, in
+ * compute_dtype. */
 int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
                             const void* sigma, const void* epsilon, long long natoms, const double* box_host,
                             const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff, double list_cutoff,

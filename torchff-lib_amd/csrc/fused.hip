@@ -36,8 +36,14 @@ constexpr int MAXEX = 8;        // exclusion partners per atom kept for the tile
 constexpr int FB = 256;         // block size of the per-atom kernels
 constexpr int TILE_BLOCK = 256; // 4 waves
 constexpr int TILE_WAVES = TILE_BLOCK / TFF_WAVE;
+constexpr int E_SLOTS = 64;  // fixed-point energy slots per term
 constexpr double F_SCALE = 4294967296.0;        // 2^32: forces and energies (kJ/mol, nm)
 constexpr double Q_SCALE = 1099511627776.0;     // 2^40: PME charge grid
+// padding slots sit at x = PAD_STEP (1 + slot), y = z = PAD_STEP: every padding slot is more
+// than PAD_STEP / 2 from every atom and from every other padding slot under any periodic
+// shift of at most one box length (boxes shorter than PAD_LIMIT)
+constexpr double PAD_STEP = 1.0e6;
+constexpr double PAD_LIMIT = 5.0e5;
 
 template <typename T> struct V4 { T x, y, z, w; };
 template <typename T> struct V2 { T x, y; };
@@ -45,9 +51,7 @@ template <typename T> struct V2 { T x, y; };
 struct Geo {
   double L[3], invL[3];
   int ncol[2];
-  int nzb;
-  double colw[2], zbw;
-  int nkeys;
+  double colw[2];
   int ncl_max;
   long long pair_cap;
   int maxn;  // cluster-pair segment length per i-cluster
@@ -57,90 +61,162 @@ struct Geo {
 __device__ __forceinline__ long long to_fx(double v, double s) { return __double2ll_rn(v * s); }
 
 // ---------------------------------------------------------------- 1. binning
+// Atoms are binned into x-y columns, the columns are scanned, the atoms scattered into their
+// column's range, and every column is sorted by (z, atom index) in LDS, so that the order
+// (and everything computed from it) does not depend on the order of the atomics.
+constexpr int COL_SORT_MAX = 8192;  // atoms per column sorted in LDS (64 KB of keys)
+
 template <typename TI>
-__global__ __launch_bounds__(FB) void k_bin_atoms(const TI* __restrict__ coords, long long n, Geo g,
-                                                  int* __restrict__ key_of, int* __restrict__ key_count) {
-  long long i = blockIdx.x * (long long)FB + threadIdx.x;
-  if (i >= n) return;
-  double w[3];
+__device__ __forceinline__ void wrap_coords(const TI* __restrict__ coords, long long i, const Geo& g, double w[3]) {
   for (int a = 0; a < 3; ++a) {
     double f = (double)coords[3 * i + a] * g.invL[a];
     f -= floor(f);
     w[a] = f * g.L[a];
   }
-  int cx = (int)(w[0] / g.colw[0]); cx = cx < 0 ? 0 : (cx >= g.ncol[0] ? g.ncol[0] - 1 : cx);
-  int cy = (int)(w[1] / g.colw[1]); cy = cy < 0 ? 0 : (cy >= g.ncol[1] ? g.ncol[1] - 1 : cy);
-  int cz = (int)(w[2] / g.zbw);     cz = cz < 0 ? 0 : (cz >= g.nzb ? g.nzb - 1 : cz);
-  int key = (cx * g.ncol[1] + cy) * g.nzb + cz;
-  key_of[i] = key;
-  atomicAdd(&key_count[key], 1);
 }
 
-// exclusive scan (single workgroup); also counts clusters per column and their offsets
-__global__ __launch_bounds__(1024) void k_scan_keys(const int* __restrict__ cnt, int m, int* __restrict__ start) {
-  __shared__ int s[1024];
-  int t = threadIdx.x;
-  int per = (m + 1023) / 1024;
-  int b = t * per, e = b + per < m ? b + per : m;
-  int sum = 0;
-  for (int k = b; k < e; ++k) sum += cnt[k];
-  s[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
+// Column of every atom and its offset within the column: the lanes of a wave that share a
+// column reserve one contiguous range with a single atomic (binning water boxes puts hundreds
+// of atoms on each column counter, so per-atom atomics would serialise).
+template <typename TI>
+__global__ __launch_bounds__(FB) void k_bin_atoms(const TI* __restrict__ coords, const TI* __restrict__ q, long long n,
+                                                  Geo g, int* __restrict__ col_of, int* __restrict__ off_in_col,
+                                                  int* __restrict__ col_count, double* __restrict__ q2part) {
+  const long long i = blockIdx.x * (long long)FB + threadIdx.x;
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const bool valid = i < n;
+  {
+    // sum of q^2 of the block (self energy), one partial per block
+    const double qi = valid ? (double)q[i] : 0.0;
+    const double t = tff_block_sum(qi * qi);
+    if (threadIdx.x == 0) q2part[blockIdx.x] = t;
   }
-  int run = t > 0 ? s[t - 1] : 0;
-  for (int k = b; k < e; ++k) {
-    start[k] = run;
-    run += cnt[k];
+  int col = -1;
+  if (valid) {
+    double w[3];
+    wrap_coords(coords, i, g, w);
+    int cx = (int)(w[0] / g.colw[0]); cx = cx < 0 ? 0 : (cx >= g.ncol[0] ? g.ncol[0] - 1 : cx);
+    int cy = (int)(w[1] / g.colw[1]); cy = cy < 0 ? 0 : (cy >= g.ncol[1] ? g.ncol[1] - 1 : cy);
+    col = cx * g.ncol[1] + cy;
   }
-  if (t == 1023) start[m] = s[1023];
+  int off = 0;
+  unsigned long long active = __ballot(valid);
+  while (active) {
+    const int leader = __ffsll((long long)active) - 1;
+    const int c0 = __shfl(col, leader, TFF_WAVE);
+    const unsigned long long same = __ballot(valid && col == c0) & active;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&col_count[c0], __popcll(same));
+    base = __shfl(base, leader, TFF_WAVE);
+    if (valid && col == c0) off = base + __popcll(same & ((1ull << lane) - 1ull));
+    active &= ~same;
+  }
+  if (valid) {
+    col_of[i] = col;
+    off_in_col[i] = off;
+  }
 }
 
-// clusters per column -> cluster offsets per column (single workgroup scan over columns)
-__global__ __launch_bounds__(1024) void k_scan_columns(const int* __restrict__ key_start, int ncols, int nzb,
-                                                       int* __restrict__ col_cl_start, int* __restrict__ n_clusters) {
-  __shared__ int s[1024];
-  int t = threadIdx.x;
-  int per = (ncols + 1023) / 1024;
-  int b = t * per, e = b + per < ncols ? b + per : ncols;
-  int sum = 0;
+// exclusive scans over the columns of their atom counts and cluster counts (one workgroup)
+__global__ __launch_bounds__(1024) void k_scan_columns(const int* __restrict__ col_count, int ncols,
+                                                       int* __restrict__ col_start, int* __restrict__ col_cl_start,
+                                                       int* __restrict__ n_clusters, int* __restrict__ status) {
+  __shared__ int sa[1024], sc[1024];
+  const int t = threadIdx.x;
+  const int per = (ncols + 1023) / 1024;
+  const int b = t * per, e = b + per < ncols ? b + per : ncols;
+  int na = 0, nc = 0;
+  bool big = false;
   for (int c = b; c < e; ++c) {
-    int na = key_start[(c + 1) * nzb] - key_start[c * nzb];
-    sum += (na + CL - 1) / CL;
+    const int m = col_count[c];
+    na += m;
+    nc += (m + CL - 1) / CL;
+    big = big || m > COL_SORT_MAX;
   }
-  s[t] = sum;
+  if (big) atomicOr(status, TFF_FUSED_COLUMN_OVERFLOW);
+  sa[t] = na;
+  sc[t] = nc;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? s[t - off] : 0;
+    const int va = t >= off ? sa[t - off] : 0, vc = t >= off ? sc[t - off] : 0;
     __syncthreads();
-    s[t] += v;
+    sa[t] += va;
+    sc[t] += vc;
     __syncthreads();
   }
-  int run = t > 0 ? s[t - 1] : 0;
+  int ra = t > 0 ? sa[t - 1] : 0, rc = t > 0 ? sc[t - 1] : 0;
   for (int c = b; c < e; ++c) {
-    col_cl_start[c] = run;
-    int na = key_start[(c + 1) * nzb] - key_start[c * nzb];
-    run += (na + CL - 1) / CL;
+    const int m = col_count[c];
+    col_start[c] = ra;
+    col_cl_start[c] = rc;
+    ra += m;
+    rc += (m + CL - 1) / CL;
   }
   if (t == 1023) {
-    col_cl_start[ncols] = s[1023];
-    *n_clusters = s[1023];
+    col_start[ncols] = sa[1023];
+    col_cl_start[ncols] = sc[1023];
+    *n_clusters = sc[1023];
   }
 }
 
-__global__ __launch_bounds__(FB) void k_scatter_atoms(long long n, const int* __restrict__ key_of,
-                                                      const int* __restrict__ key_start, int* __restrict__ cursor,
-                                                      int* __restrict__ sorted_orig, int* __restrict__ pos_of) {
+__global__ __launch_bounds__(FB) void k_scatter_atoms(long long n, const int* __restrict__ col_of,
+                                                      const int* __restrict__ off_in_col,
+                                                      const int* __restrict__ col_start, int* __restrict__ sorted_orig) {
   long long i = blockIdx.x * (long long)FB + threadIdx.x;
   if (i >= n) return;
-  int k = key_of[i];
-  int p = key_start[k] + atomicAdd(&cursor[k], 1);
-  sorted_orig[p] = (int)i;
-  pos_of[i] = p;
+  sorted_orig[col_start[col_of[i]] + off_in_col[i]] = (int)i;
+}
+
+// One workgroup per column: bitonic sort of the keys (float bits of the wrapped z << 32 | atom)
+// in LDS.  Columns above COL_SORT_MAX atoms are left in scatter order (flagged by the scan).
+template <typename TI>
+__global__ __launch_bounds__(256) void k_sort_columns(const TI* __restrict__ coords, Geo g,
+                                                      const int* __restrict__ col_start,
+                                                      int* __restrict__ sorted_orig) {
+  __shared__ unsigned long long s_k[COL_SORT_MAX];
+  const int col = blockIdx.x;
+  const int cs = col_start[col], m = col_start[col + 1] - cs;
+  if (m <= 1 || m > COL_SORT_MAX) return;
+  int np = 1;
+  while (np < m) np <<= 1;
+  for (int k = threadIdx.x; k < np; k += blockDim.x) {
+    unsigned long long key = ~0ull;
+    if (k < m) {
+      const int i = sorted_orig[cs + k];
+      double w[3];
+      wrap_coords(coords, i, g, w);
+      key = ((unsigned long long)__float_as_uint((float)w[2]) << 32) | (unsigned)i;
+    }
+    s_k[k] = key;
+  }
+  __syncthreads();
+  for (int size = 2; size <= np; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < (np >> 1); k += blockDim.x) {
+        const int lo = (k / stride) * 2 * stride + (k % stride), hi = lo + stride;
+        const bool asc = (lo & size) == 0;
+        const unsigned long long x = s_k[lo], y = s_k[hi];
+        if ((x > y) == asc) {
+          s_k[lo] = y;
+          s_k[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < m; k += blockDim.x) sorted_orig[cs + k] = (int)(unsigned)(s_k[k] & 0xffffffffull);
+}
+
+// zero several int32 buffers in one launch (lengths in 32-bit words)
+struct ClearList {
+  int* p[8];
+  long long n[8];
+  int count;
+};
+__global__ void k_clear(ClearList c) {
+  const long long s = (long long)gridDim.x * blockDim.x;
+  for (int b = 0; b < c.count; ++b)
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < c.n[b]; i += s) c.p[b][i] = 0;
 }
 
 // ---------------------------------------------------------------- 2. cluster arrays
@@ -150,7 +226,7 @@ __global__ __launch_bounds__(FB) void k_fill_clusters(const TI* __restrict__ coo
                                                       const TI* __restrict__ sig, const TI* __restrict__ eps,
                                                       Geo g, const int* __restrict__ ncl_ptr,
                                                       const int* __restrict__ col_cl_start,
-                                                      const int* __restrict__ key_start,
+                                                      const int* __restrict__ col_start,
                                                       const int* __restrict__ sorted_orig,
                                                       V4<TC>* __restrict__ xyzq, V2<TC>* __restrict__ lj,
                                                       int* __restrict__ slot_orig, int* __restrict__ cl_col) {
@@ -166,7 +242,7 @@ __global__ __launch_bounds__(FB) void k_fill_clusters(const TI* __restrict__ coo
   }
   int col = lo;
   if (s == 0) cl_col[c] = col;
-  int cs = key_start[col * g.nzb], ce = key_start[(col + 1) * g.nzb];
+  int cs = col_start[col], ce = col_start[col + 1];
   int p = cs + (c - col_cl_start[col]) * CL + s;
   if (p < ce) {
     int i = sorted_orig[p];
@@ -177,10 +253,10 @@ __global__ __launch_bounds__(FB) void k_fill_clusters(const TI* __restrict__ coo
       w[a] = f * g.L[a];
     }
     xyzq[sl] = {(TC)w[0], (TC)w[1], (TC)w[2], (TC)q[i]};
-    lj[sl] = {(TC)(0.5 * (double)sig[i]), (TC)sqrt((double)eps[i] > 0 ? (double)eps[i] : 0.0)};
+    lj[sl] = {(TC)(0.5 * (double)sig[i]), (TC)(2.0 * sqrt((double)eps[i] > 0 ? (double)eps[i] : 0.0))};
     slot_orig[sl] = i;
   } else {
-    xyzq[sl] = {(TC)0, (TC)0, (TC)0, (TC)0};
+    xyzq[sl] = {(TC)(PAD_STEP * (1.0 + sl)), (TC)PAD_STEP, (TC)PAD_STEP, (TC)0};
     lj[sl] = {(TC)0, (TC)0};
     slot_orig[sl] = -1;
   }
@@ -189,13 +265,19 @@ __global__ __launch_bounds__(FB) void k_fill_clusters(const TI* __restrict__ coo
 // Reorder the 32 slots of every cluster into four spatially compact groups of 8: sort by x
 // and split in halves, sort each half by y and split, order each group by z (padding slots,
 // keyed +inf, collect at the end; ties broken by slot).  One wave per cluster, lanes 0..31.
-// Also writes slot_of_orig[i], the final slot of atom i.
+// Also writes slot_of_orig[i] (the final slot of atom i) and the bounding boxes (centre, half
+// extent) of the cluster and of its four groups; an empty group gets half extent -1.  The
+// boxes are widened by a relative epsilon so that float rounding of the tests never drops a pair.
 template <typename TC>
 __global__ __launch_bounds__(FB) void k_reorder_clusters(const int* __restrict__ ncl_ptr, V4<TC>* __restrict__ xyzq,
                                                          V2<TC>* __restrict__ lj, int* __restrict__ slot_orig,
-                                                         int* __restrict__ slot_of_orig) {
+                                                         int* __restrict__ slot_of_orig, float4* __restrict__ bb_c,
+                                                         float4* __restrict__ bb_h, float4* __restrict__ sb_c,
+                                                         float4* __restrict__ sb_h) {
+  __shared__ float4 s_p[FB / TFF_WAVE][CL];
   const int c = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
   const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const int w = threadIdx.x / TFF_WAVE;
   if (c >= *ncl_ptr) return;
   const int s = lane & (CL - 1);
   const int sl = c * CL + s;
@@ -226,63 +308,24 @@ __global__ __launch_bounds__(FB) void k_reorder_clusters(const int* __restrict__
     gt += rt >= CL / 4;
     rz += (gt == grp && (vz < kz || (vz == kz && t < s))) ? 1 : 0;
   }
-  const int ns = c * CL + grp * 8 + rz;
+  const int nl = grp * 8 + rz;
+  const int ns = c * CL + nl;
   __builtin_amdgcn_wave_barrier();
   if (lane < CL) {
     xyzq[ns] = p;
     lj[ns] = l;
     slot_orig[ns] = o;
     if (o >= 0) slot_of_orig[o] = ns;
+    s_p[w][nl] = make_float4((float)p.x, (float)p.y, (float)p.z, o >= 0 ? 1.f : 0.f);
   }
-}
-
-// exclusion partners of every slot as packed (cluster << 5 | slot), -1 = none
-__global__ __launch_bounds__(FB) void k_fill_exclusions(long long n, const int* __restrict__ slot_of_orig,
-                                                        const int64_t* __restrict__ row_ptr,
-                                                        const int64_t* __restrict__ col,
-                                                        int* __restrict__ excl_part, int* __restrict__ status) {
-  long long i = blockIdx.x * (long long)FB + threadIdx.x;
-  if (i >= n) return;
-  int si = slot_of_orig[i];
-  int k = 0;
-  if (col) {
-    for (long long e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
-      long long j = col[e];
-      if (j == i || j < 0 || j >= n) continue;
-      if (k == MAXEX) {
-        atomicOr(status, TFF_FUSED_EXCL_OVERFLOW);
-        break;
-      }
-      excl_part[si * MAXEX + k++] = slot_of_orig[j];
-    }
-  }
-  for (; k < MAXEX; ++k) excl_part[si * MAXEX + k] = -1;
-}
-
-// padding slots carry no exclusions
-__global__ void k_clear_pad_exclusions(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
-                                       int* __restrict__ excl_part) {
-  int sl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= *ncl_ptr * CL) return;
-  if (slot_orig[sl] < 0)
-    for (int k = 0; k < MAXEX; ++k) excl_part[sl * MAXEX + k] = -1;
-}
-
-// cluster bounding boxes (centre, half extent) of the whole cluster and of its four groups of
-// 8 slots; an empty group gets half extent -1.  One wave per cluster, lanes 0..31.
-template <typename TC>
-__global__ __launch_bounds__(FB) void k_bbox(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
-                                             const int* __restrict__ slot_orig, float4* __restrict__ bb_c,
-                                             float4* __restrict__ bb_h, float4* __restrict__ sb_c,
-                                             float4* __restrict__ sb_h) {
-  int c = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
-  int lane = threadIdx.x & (TFF_WAVE - 1);
-  if (c >= *ncl_ptr) return;
-  int sl = c * CL + (lane & (CL - 1));
-  bool v = slot_orig[sl] >= 0;
-  V4<TC> p = xyzq[sl];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // bounding boxes of the reordered slots: lane l holds new slot l & 31
+  const float4 q = s_p[w][lane & (CL - 1)];
+  const bool v = q.w != 0.f;
   float mn[3], mx[3], gmn[3], gmx[3];
-  float pv[3] = {(float)p.x, (float)p.y, (float)p.z};
+  const float pv[3] = {q.x, q.y, q.z};
   for (int a = 0; a < 3; ++a) {
     mn[a] = v ? pv[a] : 3.0e38f;
     mx[a] = v ? pv[a] : -3.0e38f;
@@ -297,7 +340,6 @@ __global__ __launch_bounds__(FB) void k_bbox(const int* __restrict__ ncl_ptr, co
       mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, TFF_WAVE));
     }
   }
-  // widen by a relative epsilon so float rounding of the tests never drops a pair
   if (lane == 0) {
     bb_c[c] = make_float4(0.5f * (mn[0] + mx[0]), 0.5f * (mn[1] + mx[1]), 0.5f * (mn[2] + mx[2]), 0.f);
     bb_h[c] = make_float4(0.5f * (mx[0] - mn[0]) * 1.0001f + 1e-6f, 0.5f * (mx[1] - mn[1]) * 1.0001f + 1e-6f,
@@ -316,17 +358,59 @@ __global__ __launch_bounds__(FB) void k_bbox(const int* __restrict__ ncl_ptr, co
   }
 }
 
+// exclusion partners of every slot as packed (cluster << 5 | slot), -1 = none; padding slots
+// have none
+__global__ __launch_bounds__(FB) void k_fill_exclusions(const int* __restrict__ ncl_ptr,
+                                                        const int* __restrict__ slot_orig,
+                                                        const int* __restrict__ slot_of_orig, long long n,
+                                                        const int64_t* __restrict__ row_ptr,
+                                                        const int64_t* __restrict__ col,
+                                                        int* __restrict__ excl_part, int* __restrict__ status) {
+  const int sl = blockIdx.x * FB + threadIdx.x;
+  if (sl >= *ncl_ptr * CL) return;
+  const int i = slot_orig[sl];
+  int k = 0;
+  if (i >= 0 && col) {
+    for (long long e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+      const long long j = col[e];
+      if (j == i || j < 0 || j >= n) continue;
+      if (k == MAXEX) {
+        atomicOr(status, TFF_FUSED_EXCL_OVERFLOW);
+        break;
+      }
+      excl_part[sl * MAXEX + k++] = slot_of_orig[j];
+    }
+  }
+  for (; k < MAXEX; ++k) excl_part[sl * MAXEX + k] = -1;
+}
+
 // ---------------------------------------------------------------- 3. cluster pairs
+// Cluster-pair entries are (ci, cj, info) with cj >= ci; info packs
+//   bits  0..15  sub-tile mask: bit gi * 4 + gj set when groups gi of ci and gj of cj have
+//                bounding boxes within the list cutoff (diagonal tiles: gj >= gi only)
+//   bits 16..21  periodic shift k_a + 1 (2 bits per axis): x_j + k L is the image of cluster
+//                cj closest to cluster ci
+//   bit  22      the tile holds excluded pairs
+//   bit  23      the shift is not the minimum image of every atom pair of the tile (small
+//                boxes): the pair kernel then takes the minimum image per pair
+//   bit  24      always set
+// The sub-tile masks are filled in by k_compact_pairs; entries whose mask stays empty are
+// skipped by the pair kernel.
+constexpr unsigned INFO_EXCL = 1u << 22;
+constexpr unsigned INFO_MINIMG = 1u << 23;
+constexpr unsigned INFO_LISTED = 1u << 24;  // set for every entry (so that info != 0)
+
 // One wave per i-cluster; lanes sweep the clusters of the neighbouring columns in a fixed
-// order and write the accepted j-clusters (cj >= ci) into the i-cluster's own segment.
+// order and write the accepted j-clusters into the i-cluster's own segment.
 __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ ncl_ptr, const int* __restrict__ cl_col,
                                                       const int* __restrict__ col_cl_start,
                                                       const float4* __restrict__ bb_c,
                                                       const float4* __restrict__ bb_h, Geo g, float rlist2,
                                                       const int* __restrict__ excl_part,
-                                                      int* __restrict__ seg, int* __restrict__ seg_count,
+                                                      int2* __restrict__ seg, int* __restrict__ seg_count,
                                                       int* __restrict__ status) {
   __shared__ int s_xp[FB / TFF_WAVE][CL * MAXEX];
+  __shared__ int s_off[FB / TFF_WAVE][TFF_WAVE], s_cs[FB / TFF_WAVE][TFF_WAVE];
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int w = threadIdx.x / TFF_WAVE;
   const int ci = blockIdx.x * (FB / TFF_WAVE) + w;
@@ -347,48 +431,87 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int col = cl_col[ci];
   const int cx = col / g.ncol[1], cy = col % g.ncol[1];
-  const float4 c0 = bb_c[ci], h0 = bb_h[ci];
-  const float Lx = (float)g.L[0], Ly = (float)g.L[1], Lz = (float)g.L[2];
-  const float iLx = (float)g.invL[0], iLy = (float)g.invL[1], iLz = (float)g.invL[2];
+  const float4 c0v = bb_c[ci], h0 = bb_h[ci];
+  const float Lv[3] = {(float)g.L[0], (float)g.L[1], (float)g.L[2]};
+  const float iLv[3] = {(float)g.invL[0], (float)g.invL[1], (float)g.invL[2]};
   int nseg = 0;
-  int* my = seg + (long long)ci * g.maxn;
+  int2* my = seg + (long long)ci * g.maxn;
   const int nx = 2 * g.reach[0] + 1 >= g.ncol[0] ? g.ncol[0] : 2 * g.reach[0] + 1;
   const int ny = 2 * g.reach[1] + 1 >= g.ncol[1] ? g.ncol[1] : 2 * g.reach[1] + 1;
   const int x0 = nx == g.ncol[0] ? 0 : cx - g.reach[0];
   const int y0 = ny == g.ncol[1] ? 0 : cy - g.reach[1];
-  for (int ix = 0; ix < nx; ++ix) {
-    int xx = x0 + ix;
-    xx += xx < 0 ? g.ncol[0] : (xx >= g.ncol[0] ? -g.ncol[0] : 0);
-    for (int iy = 0; iy < ny; ++iy) {
-      int yy = y0 + iy;
+  const int ncc = nx * ny;
+  // Candidates: the neighbouring columns (up to 64 at a time, one per lane) clipped to cj >= ci,
+  // scanned into one flat range that the wave walks 64 clusters at a time (column-major order).
+  for (int c0 = 0; c0 < ncc; c0 += TFF_WAVE) {
+    const int cc = c0 + lane;
+    int cs = 0, cnt = 0;
+    if (cc < ncc) {
+      int xx = x0 + cc / ny, yy = y0 + cc % ny;
+      xx += xx < 0 ? g.ncol[0] : (xx >= g.ncol[0] ? -g.ncol[0] : 0);
       yy += yy < 0 ? g.ncol[1] : (yy >= g.ncol[1] ? -g.ncol[1] : 0);
-      int ncol_id = xx * g.ncol[1] + yy;
-      int cs = col_cl_start[ncol_id], ce = col_cl_start[ncol_id + 1];
-      for (int b = cs; b < ce; b += TFF_WAVE) {
-        int cj = b + lane;
-        bool ok = false;
-        if (cj < ce && cj >= ci) {
-          float4 c1 = bb_c[cj], h1 = bb_h[cj];
-          float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
-          dx -= Lx * rintf(dx * iLx);
-          dy -= Ly * rintf(dy * iLy);
-          dz -= Lz * rintf(dz * iLz);
-          float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
-          float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
-          float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
-          ok = gx * gx + gy * gy + gz * gz <= rlist2;
-        }
-        unsigned long long m = __ballot(ok);
-        if (ok) {
-          // bit 30 marks tiles that contain excluded pairs
-          int xf = cj == ci ? 1 : 0;
-          for (int t = 0; t < nxp; ++t) xf |= s_xp[w][t] == cj ? 1 : 0;
-          int k = nseg + __popcll(m & ((1ull << lane) - 1ull));
-          if (k < g.maxn) my[k] = cj | (xf << 30);
-        }
-        nseg += __popcll(m);
-      }
+      const int col_id = xx * g.ncol[1] + yy;
+      const int ce = col_cl_start[col_id + 1];
+      cs = col_cl_start[col_id];
+      cs = cs > ci ? cs : ci;
+      cnt = ce > cs ? ce - cs : 0;
     }
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < TFF_WAVE; off <<= 1) {
+      const int v = __shfl_up(incl, off, TFF_WAVE);
+      if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, TFF_WAVE - 1, TFF_WAVE);
+    s_off[w][lane] = incl - cnt;
+    s_cs[w][lane] = cs;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int nvalid = ncc - c0 < TFF_WAVE ? ncc - c0 : TFF_WAVE;
+    for (int b = 0; b < total; b += TFF_WAVE) {
+      const int k = b + lane;
+      unsigned info = 0;
+      int cj = -1;
+      if (k < total) {
+        // last column whose offset is <= k (offsets are non-decreasing)
+        int lo = 0, hi = nvalid - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_off[w][mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        cj = s_cs[w][lo] + (k - s_off[w][lo]);
+        const float4 c1 = bb_c[cj], h1 = bb_h[cj];
+        const float d[3] = {c0v.x - c1.x, c0v.y - c1.y, c0v.z - c1.z};
+        const float hs[3] = {h0.x + h1.x, h0.y + h1.y, h0.z + h1.z};
+        float gap2 = 0.f;
+        bool minimg = false;
+        unsigned shift = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float kk = rintf(d[a] * iLv[a]);
+          const float dm = d[a] - Lv[a] * kk;
+          const float gp = fmaxf(fabsf(dm) - hs[a], 0.f);
+          gap2 += gp * gp;
+          minimg = minimg || !(fabsf(dm) + hs[a] <= 0.49999f * Lv[a]);
+          shift |= (unsigned)((int)kk + 1) << (16 + 2 * a);
+        }
+        if (gap2 <= rlist2) info = shift | (minimg ? INFO_MINIMG : 0u) | INFO_LISTED;
+      }
+      const bool ok = info != 0;
+      unsigned long long bm = __ballot(ok);
+      if (ok) {
+        bool xf = cj == ci;
+        for (int t = 0; t < nxp; ++t) xf = xf || s_xp[w][t] == cj;
+        int kk = nseg + __popcll(bm & ((1ull << lane) - 1ull));
+        if (kk < g.maxn) my[kk] = make_int2(cj, (int)(info | (xf ? INFO_EXCL : 0u)));
+      }
+      nseg += __popcll(bm);
+    }
+    // the next group of columns rewrites s_off / s_cs
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   if (lane == 0) {
     if (nseg > g.maxn) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
@@ -426,66 +549,133 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ nc
   }
 }
 
-// segments -> one compact cluster-pair list in (ci, segment order): deterministic
-__global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ ncl_ptr, const int* __restrict__ seg,
+// Segments -> one compact cluster-pair list in (ci, segment order) (deterministic), with the
+// sub-tile masks: 4 tiles per wave step, 16 lanes per tile, lane (gi, gj) testing the bounding
+// boxes of group gi of ci and group gj of cj against cut2 (minimum image of the group centres:
+// a lower bound on the distance of every image of every atom pair of the two groups).
+__global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ ncl_ptr, const int2* __restrict__ seg,
                                                       const int* __restrict__ seg_count,
                                                       const int* __restrict__ seg_start, int maxn,
-                                                      int2* __restrict__ pairs) {
+                                                      const float4* __restrict__ sb_c, const float4* __restrict__ sb_h,
+                                                      Geo g, float cut2, int4* __restrict__ pairs) {
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int ci = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
   if (ci >= *ncl_ptr) return;
   const int n = seg_count[ci], o = seg_start[ci];
-  for (int k = lane; k < n; k += TFF_WAVE) pairs[o + k] = make_int2(ci, seg[(long long)ci * maxn + k]);
-}
-
-// order the atoms of each sort key by their original index (the scatter used atomics)
-__global__ __launch_bounds__(FB) void k_sort_keys(int nkeys, const int* __restrict__ key_start,
-                                                  int* __restrict__ sorted_orig, int* __restrict__ pos_of) {
-  int k = blockIdx.x * FB + threadIdx.x;
-  if (k >= nkeys) return;
-  const int b = key_start[k], e = key_start[k + 1];
-  for (int p = b + 1; p < e; ++p) {
-    int v = sorted_orig[p];
-    int q = p - 1;
-    while (q >= b && sorted_orig[q] > v) {
-      sorted_orig[q + 1] = sorted_orig[q];
-      --q;
+  const int q = lane & 15, gi = q >> 2, gj = q & 3, sub = lane >> 4;
+  const float4 c0 = sb_c[ci * 4 + gi], h0 = sb_h[ci * 4 + gi];
+  const float L0 = (float)g.L[0], L1 = (float)g.L[1], L2 = (float)g.L[2];
+  const float i0 = (float)g.invL[0], i1 = (float)g.invL[1], i2 = (float)g.invL[2];
+  for (int k0 = 0; k0 < n; k0 += 4) {
+    const int k = k0 + sub;
+    bool ok = false;
+    int2 e = make_int2(0, 0);
+    if (k < n) {
+      e = seg[(long long)ci * maxn + k];
+      const int cj = e.x;
+      const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
+      if (h0.x >= 0.f && h1.x >= 0.f && (ci != cj || gj >= gi)) {
+        float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
+        dx -= L0 * rintf(dx * i0);
+        dy -= L1 * rintf(dy * i1);
+        dz -= L2 * rintf(dz * i2);
+        const float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
+        const float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
+        const float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
+        ok = gx * gx + gy * gy + gz * gz <= cut2;
+      }
     }
-    sorted_orig[q + 1] = v;
+    const unsigned long long bm = __ballot(ok);
+    if (k < n && q == 0) {
+      const unsigned m = (unsigned)((bm >> (lane & 48)) & 0xffffull);
+      pairs[o + k] = make_int4(ci, e.x, (int)((unsigned)e.y | m), 0);
+    }
   }
-  for (int p = b; p < e; ++p) pos_of[sorted_orig[p]] = p;
 }
 
 // ---------------------------------------------------------------- 4. tile kernel
-// erfc(u) and exp(-u^2) for u >= 0.  float: library exp, hardware reciprocal, and
-// erfc(u) exp(u^2) = sum_k c_k t^k, t = 1 / (1 + 0.39 u), a degree-8 polynomial fitted to the
-// scaled complementary error function on [0, 3.6] by iteratively reweighted least squares
-// (relative error 1.3e-9 in exact arithmetic, about 1.5e-6 evaluated in float32, against
-// scipy.special.erfc); double: the correctly rounded library functions.
+// erfc(u) and exp(-u^2) for u >= 0.
+// float: exp(-u^2) = 2^(-u^2 log2 e) with the product split into a rounded part and its fma
+// remainder (plus the rounding of log2 e in float), the rounded part through the hardware
+// base-2 exponential and the remainder as a first-order factor; erfc(u) exp(u^2) =
+// sum_k c_k t^k, t = 1 / (1 + 0.39 u) (hardware reciprocal), a degree-8 polynomial fitted
+// to the scaled complementary error function on [0, 3.6] by iteratively reweighted least
+// squares (relative error 1.3e-9 in exact arithmetic, about 1.5e-6 evaluated in float32,
+// against scipy.special.erfc).  double: the correctly rounded library functions.
 __device__ __forceinline__ void erfc_exp(float u, float& ec, float& em) {
-  em = expf(-u * u);
-  const float t = __frcp_rn(1.0f + 0.39f * u);
+  const float v = u * u;
+  const float hi = v * -1.44269502f;
+  float lo = fmaf(v, -1.44269502f, -hi);
+  lo = fmaf(v, -1.92596299e-8f, lo);
+  const float e = __builtin_amdgcn_exp2f(hi);
+  em = fmaf(e, lo * 0.693147181f, e);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.39f, u, 1.0f));
   float p = 9.27117256e-02f;
-  p = p * t - 5.29840954e-01f;
-  p = p * t + 1.06552963e+00f;
-  p = p * t - 8.61721388e-01f;
-  p = p * t + 7.81784359e-01f;
-  p = p * t - 4.21555312e-02f;
-  p = p * t + 2.82117988e-01f;
-  p = p * t + 2.10995424e-01f;
-  p = p * t + 5.78745155e-04f;
+  p = fmaf(p, t, -5.29840954e-01f);
+  p = fmaf(p, t, 1.06552963e+00f);
+  p = fmaf(p, t, -8.61721388e-01f);
+  p = fmaf(p, t, 7.81784359e-01f);
+  p = fmaf(p, t, -4.21555312e-02f);
+  p = fmaf(p, t, 2.82117988e-01f);
+  p = fmaf(p, t, 2.10995424e-01f);
+  p = fmaf(p, t, 5.78745155e-04f);
   ec = p * em;
 }
 __device__ __forceinline__ void erfc_exp(double u, double& ec, double& em) {
   em = exp(-u * u);
   ec = erfc(u);
 }
-// hardware reciprocal square root refined by one Newton step (LJ raises its error to the 6th power)
+// hardware reciprocal square root refined by one Newton step (Lennard-Jones raises its error
+// to the sixth power)
 __device__ __forceinline__ float rsqrt_t(float x) {
-  const float y = rsqrtf(x);
-  return y * (1.5f - 0.5f * x * y * y);
+  const float y = __builtin_amdgcn_rsqf(x);
+  return y * fmaf(-0.5f * x, y * y, 1.5f);
 }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+
+// sum over the 8 lanes sharing lane >> 3 (the result lands in all 8 lanes)
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float octet_sum(float v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  return v;
+}
+__device__ __forceinline__ double octet_sum(double v) {
+  v += __shfl_xor(v, 1, TFF_WAVE);
+  v += __shfl_xor(v, 2, TFF_WAVE);
+  v += __shfl_xor(v, 4, TFF_WAVE);
+  return v;
+}
+// sum over the 8 lanes sharing lane & 7
+template <typename T>
+__device__ __forceinline__ T stride8_sum(T v) {
+  v += __shfl_xor(v, 8, TFF_WAVE);
+  v += __shfl_xor(v, 16, TFF_WAVE);
+  v += __shfl_xor(v, 32, TFF_WAVE);
+  return v;
+}
+
+// The 12 force components f[g][a] (k = 3 g + a) of one slot column are summed over the 8 lanes
+// that share the column (RED) and written by those 8 lanes: lane r of the eight takes values
+// r and r + 8.  The selects act on the reduced values, so the arrays stay in registers.
+template <typename T, typename RED>
+__device__ __forceinline__ void reduce_pick2(const T (&f)[4][3], int r, RED red, T& v0, T& v1) {
+  const T k0 = red(f[0][0]), k1 = red(f[0][1]), k2 = red(f[0][2]), k3 = red(f[1][0]);
+  const T k4 = red(f[1][1]), k5 = red(f[1][2]), k6 = red(f[2][0]), k7 = red(f[2][1]);
+  const T k8 = red(f[2][2]), k9 = red(f[3][0]), k10 = red(f[3][1]), k11 = red(f[3][2]);
+  const bool b0 = r & 1, b1 = r & 2, b2 = r & 4;
+  const T s0 = b0 ? k1 : k0, s1 = b0 ? k3 : k2, s2 = b0 ? k5 : k4, s3 = b0 ? k7 : k6;
+  const T t0 = b1 ? s1 : s0, t1 = b1 ? s3 : s2;
+  v0 = b2 ? t1 : t0;
+  const T w0 = b0 ? k9 : k8, w1 = b0 ? k11 : k10;
+  v1 = b1 ? w1 : w0;
+}
+// offset of value k of a slot column within its cluster's 96 accumulators (slot = g * 8 + col)
+__device__ __forceinline__ int acc_offset(int k, int col) { return 24 * (k / 3) + 3 * col + k % 3; }
 
 template <typename TC>
 struct TileArgs {
@@ -495,201 +685,198 @@ struct TileArgs {
   TC coul;
 };
 
-// One wave per 32 x 32 cluster tile at a time, evaluated as up to 16 sub-tiles of 8 x 8 atoms
-// (the clusters' compact groups of 8): lane = (ii = lane & 7, jj = lane >> 3) holds one pair of
-// a sub-tile, and only sub-tiles whose group bounding boxes are within the list cutoff are
-// evaluated (a wave-uniform loop).  F_i stays in registers across consecutive tiles of the same
-// i-cluster; F_j is reduced over the 8 lanes sharing jj and added with int64 atomics per tile.
-template <typename TC>
-__global__ __launch_bounds__(TILE_BLOCK) void k_tiles(const int2* __restrict__ cpairs, const int* __restrict__ ncp_ptr,
-                                                      long long cap, const V4<TC>* __restrict__ xyzq,
-                                                      const V2<TC>* __restrict__ lj, const int* __restrict__ slot_orig,
-                                                      const int* __restrict__ excl_part, const float4* __restrict__ sb_c,
-                                                      const float4* __restrict__ sb_h, TileArgs<TC> ta, float rlist2,
-                                                      unsigned long long* __restrict__ facc,
-                                                      unsigned long long* __restrict__ eacc) {
-  const int lane = threadIdx.x & (TFF_WAVE - 1);
-  const int w = threadIdx.x / TFF_WAVE;
-  const int ii = lane & 7, jj = lane >> 3;
-  long long ncp = *ncp_ptr;
-  if (ncp > cap) ncp = cap;
-  const long long nw = (long long)gridDim.x * TILE_WAVES;
-  const long long gw = (long long)blockIdx.x * TILE_WAVES + w;
-  const long long per = (ncp + nw - 1) / nw;
-  const long long t0 = gw * per;
-  long long t1 = t0 + per;
-  if (t1 > ncp) t1 = ncp;
-  const TC two_over_sqrt_pi = (TC)1.1283791670955126;
-  const float fL[3] = {(float)ta.L[0], (float)ta.L[1], (float)ta.L[2]};
-  const float fiL[3] = {(float)ta.invL[0], (float)ta.invL[1], (float)ta.invL[2]};
-  TC e_lj = 0, e_c = 0;
-  int cur_i = -1;
-  TC fi[4][3];
-  V4<TC> pi[4];
-  V2<TC> li[4];
-  bool vi[4];
+// Up to 16 sub-tiles of 8 x 8 atoms of one cluster-pair tile.  Lane (ii = lane & 7, jj = lane >> 3)
+// evaluates pair (slot gi * 8 + ii of ci, slot gj * 8 + jj of cj) of sub-tile b = gi * 4 + gj.
+// MINIMG: minimum image per pair and explicit slot validity (else pj is already shifted to
+// the nearest image and padding slots sit far away); EXCL: the tile holds excluded pairs.
+template <typename TC, bool MINIMG, bool EXCL>
+__device__ __forceinline__ void tile_steps(unsigned mask, bool upper, const V4<TC> (&pi)[4], const V2<TC> (&li)[4],
+                                           const V4<TC> (&pj)[4], const V2<TC> (&lJ)[4], const unsigned (&exm)[4],
+                                           int jj, const TileArgs<TC>& ta, TC (&fi)[4][3], TC (&fj)[4][3], TC& e_c,
+                                           TC& e_lj) {
+  const TC c2 = (TC)1.1283791670955126;  // 2 / sqrt(pi)
+  // j-group major, so that the first steps wait only for the first j-group's loads
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    fi[g][0] = fi[g][1] = fi[g][2] = 0;
-    vi[g] = false;
-  }
-  auto flush_i = [&](int c) {
-    // reduce over the 8 lanes sharing ii (lane bits 3..5), lanes 0..7 write
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        TC v = fi[g][a];
-        v += __shfl_xor(v, 8, TFF_WAVE);
-        v += __shfl_xor(v, 16, TFF_WAVE);
-        v += __shfl_xor(v, 32, TFF_WAVE);
-        fi[g][a] = v;
-      }
-    if (jj == 0) {
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int sl = c * CL + g * 8 + ii;
-        if (vi[g]) {
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-            if (fi[g][a] != (TC)0) atomicAdd(&facc[3 * sl + a], (unsigned long long)to_fx((double)fi[g][a], F_SCALE));
-        }
-      }
-    }
-  };
-  for (long long t = t0; t < t1; ++t) {
-    const int2 cp = cpairs[t];
-    const int ci = cp.x, cj = cp.y & 0x3fffffff;
-    const bool xflag = (cp.y >> 30) & 1;
-    if (ci != cur_i) {
-      if (cur_i >= 0) flush_i(cur_i);
-      cur_i = ci;
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int sl = ci * CL + g * 8 + ii;
-        pi[g] = xyzq[sl];
-        li[g] = lj[sl];
-        vi[g] = slot_orig[sl] >= 0;
-        fi[g][0] = fi[g][1] = fi[g][2] = 0;
-      }
-    }
-    const bool diag = ci == cj;
-    // active sub-tiles: group bounding boxes within the list cutoff (lanes 0..15 test one each)
-    bool sub_ok = false;
-    if (lane < 16) {
-      const int gi = lane >> 2, gj = lane & 3;
-      const float4 c0 = sb_c[ci * 4 + gi], h0 = sb_h[ci * 4 + gi];
-      const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
-      if (h0.x >= 0.f && h1.x >= 0.f && (!diag || gj >= gi)) {
-        float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
-        dx -= fL[0] * rintf(dx * fiL[0]);
-        dy -= fL[1] * rintf(dy * fiL[1]);
-        dz -= fL[2] * rintf(dz * fiL[2]);
-        const float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
-        const float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
-        const float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
-        sub_ok = gx * gx + gy * gy + gz * gz <= rlist2;
-      }
-    }
-    const unsigned int mask = (unsigned int)(__ballot(sub_ok) & 0xffffull);
-    if (mask == 0) continue;
-    // exclusion bit masks of this lane's four i-slots against cj (only for flagged tiles)
-    unsigned int exm[4] = {0u, 0u, 0u, 0u};
-    if (xflag) {
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int sl = ci * CL + g * 8 + ii;
-#pragma unroll
-        for (int k = 0; k < MAXEX; ++k) {
-          const int pv = excl_part[sl * MAXEX + k];
-          if (pv >= 0 && (pv >> 5) == cj) exm[g] |= 1u << (pv & 31);
-        }
-      }
-    }
-    V4<TC> pj[4];
-    V2<TC> lJ[4];
-    bool vj[4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int sl = cj * CL + g * 8 + jj;
-      pj[g] = xyzq[sl];
-      lJ[g] = lj[sl];
-      vj[g] = slot_orig[sl] >= 0;
-    }
-    TC fj[4][3];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) fj[g][0] = fj[g][1] = fj[g][2] = 0;
-#pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      if (!((mask >> b) & 1u)) continue;
-      const int gi = b >> 2, gj = b & 3;
-      TC dx = pi[gi].x - pj[gj].x, dy = pi[gi].y - pj[gj].y, dz = pi[gi].z - pj[gj].z;
+  for (int b = 0; b < 16; ++b) {
+    const int gj = b >> 2, gi = b & 3;
+    if (!((mask >> (gi * 4 + gj)) & 1u)) continue;
+    TC dx = pi[gi].x - pj[gj].x, dy = pi[gi].y - pj[gj].y, dz = pi[gi].z - pj[gj].z;
+    if (MINIMG) {
       dx -= ta.L[0] * rint(dx * ta.invL[0]);
       dy -= ta.L[1] * rint(dy * ta.invL[1]);
       dz -= ta.L[2] * rint(dz * ta.invL[2]);
-      const TC r2 = dx * dx + dy * dy + dz * dz;
-      const bool ok = vi[gi] && vj[gj] && r2 <= ta.rc2 && (!(diag && gi == gj) || jj > ii);
-      TC fs = 0;
-      if (ok) {
-        const bool excl = (exm[gi] >> (gj * 8 + jj)) & 1u;
-        const TC rinv = rsqrt_t(r2);
-        const TC rinv2 = rinv * rinv;
-        const TC r = r2 * rinv;
-        const TC u = ta.alpha * r;
-        TC ec, em;
-        erfc_exp(u, ec, em);
-        ec -= excl ? (TC)1 : (TC)0;
-        const TC ex2 = em * two_over_sqrt_pi * u;
-        const TC qq = ta.coul * pi[gi].w * pj[gj].w;
-        e_c += qq * ec * rinv;
-        fs = qq * (ec + ex2) * rinv * rinv2;
-        if (!excl) {
-          const TC sg = li[gi].x + lJ[gj].x;
-          const TC ep = li[gi].y * lJ[gj].y;
-          const TC s2 = sg * sg * rinv2;
-          const TC s6 = s2 * s2 * s2;
-          e_lj += (TC)4 * ep * s6 * (s6 - (TC)1);
-          fs += (TC)24 * ep * s6 * ((TC)2 * s6 - (TC)1) * rinv2;
+    }
+    const TC r2 = dx * dx + dy * dy + dz * dz;
+    bool ok = r2 <= ta.rc2;
+    if (MINIMG) ok = ok && pi[gi].x < (TC)PAD_LIMIT && pj[gj].x < (TC)PAD_LIMIT;
+    if (gi == gj) ok = ok && upper;
+    bool excl = false;
+    if (EXCL) excl = (exm[gi] >> (gj * 8 + jj)) & 1u;
+    const TC rinv = rsqrt_t(r2);
+    const TC rinv2 = rinv * rinv;
+    const TC u = ta.alpha * (r2 * rinv);
+    TC ec, em;
+    erfc_exp(u, ec, em);
+    if (EXCL) ec = excl ? ec - (TC)1 : ec;
+    const TC qr = pi[gi].w * pj[gj].w * rinv;
+    const TC ecr = qr * ec;
+    TC fs = qr * fma(c2 * u, em, ec) * rinv2;
+    TC ep4 = li[gi].y * lJ[gj].y;
+    if (EXCL) ep4 = excl ? (TC)0 : ep4;
+    const TC sg = li[gi].x + lJ[gj].x;
+    const TC s2 = sg * sg * rinv2;
+    const TC s6 = s2 * s2 * s2;
+    const TC elj = ep4 * fma(s6, s6, -s6);
+    fs = fma(ep4 * s6 * fma((TC)12, s6, (TC)-6), rinv2, fs);
+    fs = ok ? fs : (TC)0;
+    e_c += ok ? ecr : (TC)0;
+    e_lj += ok ? elj : (TC)0;
+    // F_i += fs d, F_j -= fs d   (fs = -dE/dr / r)
+    fi[gi][0] = fma(fs, dx, fi[gi][0]);
+    fi[gi][1] = fma(fs, dy, fi[gi][1]);
+    fi[gi][2] = fma(fs, dz, fi[gi][2]);
+    fj[gj][0] = fma(-fs, dx, fj[gj][0]);
+    fj[gj][1] = fma(-fs, dy, fj[gj][1]);
+    fj[gj][2] = fma(-fs, dz, fj[gj][2]);
+  }
+}
+
+// One wave per workgroup walks a contiguous run of the (ci-ordered) tile list (the dispatcher
+// starts a new wave as soon as one finishes, which balances the uneven tiles).  F_i stays in
+// registers while ci repeats; F_j of every tile is summed over the 8 lanes sharing jj and added
+// to the int64 fixed-point accumulators.  The wave's energies go to its own slot of a partial
+// array, summed in a fixed order afterwards, so the results are bitwise reproducible.  Charges
+// are pre-multiplied by the Coulomb constant on the i side; lj holds (sigma / 2, 2 sqrt(eps))
+// so that li.y * lJ.y = 4 eps_ij.
+template <typename T>
+__device__ __forceinline__ T wave_sum_t(T v) {
+  v = octet_sum(v);
+  return stride8_sum(v);
+}
+
+template <typename TC>
+__device__ __forceinline__ void tiles_body(const int4* __restrict__ cpairs, const int* __restrict__ ncp_ptr,
+                                           long long cap, const V4<TC>* __restrict__ xyzq,
+                                           const V2<TC>* __restrict__ lj, const int* __restrict__ excl_part,
+                                           const TileArgs<TC>& ta, unsigned long long* __restrict__ facc,
+                                           unsigned long long* __restrict__ eacc) {
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const int ii = lane & 7, jj = lane >> 3;
+  long long ncp = *ncp_ptr;
+  if (ncp > cap) ncp = cap;
+  const long long nw = gridDim.x;
+  const long long gw = blockIdx.x;
+  const long long per = (ncp + nw - 1) / nw;
+  const long long t0 = gw * per;
+  const long long t1 = t0 + per < ncp ? t0 + per : ncp;
+  // accumulator offsets of the two values this lane writes for a j column (jj) / an i column (ii)
+  const int oj0 = acc_offset(ii, jj), oj1 = acc_offset(ii + 8 < 12 ? ii + 8 : 11, jj);
+  const int oi0 = acc_offset(jj, ii), oi1 = acc_offset(jj + 8 < 12 ? jj + 8 : 11, ii);
+  TC fi[4][3];
+  V4<TC> pi[4];
+  V2<TC> li[4];
+  auto flush_i = [&](int c) {
+    TC v0, v1;
+    reduce_pick2(fi, jj, [](TC v) { return stride8_sum(v); }, v0, v1);
+    unsigned long long* base = facc + (long long)c * (3 * CL);
+    if (v0 != (TC)0) atomicAdd(base + oi0, (unsigned long long)to_fx((double)v0, F_SCALE));
+    if (jj < 4 && v1 != (TC)0) atomicAdd(base + oi1, (unsigned long long)to_fx((double)v1, F_SCALE));
+  };
+  TC e_lj = 0, e_c = 0;
+  int cur_i = -1;
+  {
+    for (long long t = t0; t < t1; ++t) {
+      const int4 cp = cpairs[t];
+      const int ci = cp.x, cj = cp.y;
+      const unsigned info = (unsigned)cp.z;
+      const unsigned mask = info & 0xffffu;
+      if (mask == 0) continue;
+      if (ci != cur_i) {
+        if (cur_i >= 0) flush_i(cur_i);
+        cur_i = ci;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int sl = ci * CL + g * 8 + ii;
+          pi[g] = xyzq[sl];
+          pi[g].w *= ta.coul;
+          li[g] = lj[sl];
+          fi[g][0] = fi[g][1] = fi[g][2] = 0;
         }
       }
-      // F_i += fs d, F_j -= fs d   (fs = -dE/dr / r)
-      fi[gi][0] += fs * dx;
-      fi[gi][1] += fs * dy;
-      fi[gi][2] += fs * dz;
-      fj[gj][0] -= fs * dx;
-      fj[gj][1] -= fs * dy;
-      fj[gj][2] -= fs * dz;
-    }
-    // reduce F_j over the 8 lanes sharing jj (lane bits 0..2); lanes with ii == 0 write
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        TC v = fj[g][a];
-        v += __shfl_xor(v, 1, TFF_WAVE);
-        v += __shfl_xor(v, 2, TFF_WAVE);
-        v += __shfl_xor(v, 4, TFF_WAVE);
-        fj[g][a] = v;
-      }
-    if (ii == 0) {
+      const bool upper = ci != cj || jj > ii;
+      V4<TC> pj[4];
+      V2<TC> lJ[4];
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         const int sl = cj * CL + g * 8 + jj;
-        if (vj[g]) {
+        pj[g] = xyzq[sl];
+        lJ[g] = lj[sl];
+      }
+      unsigned exm[4] = {0u, 0u, 0u, 0u};
+      if (info & INFO_EXCL) {
+        // exclusion bit masks of this lane's four i-slots against the slots of cj
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
-            if (fj[g][a] != (TC)0) atomicAdd(&facc[3 * sl + a], (unsigned long long)to_fx((double)fj[g][a], F_SCALE));
+        for (int g = 0; g < 4; ++g) {
+          const int sl = ci * CL + g * 8 + ii;
+#pragma unroll
+          for (int k = 0; k < MAXEX; ++k) {
+            const int pv = excl_part[sl * MAXEX + k];
+            if (pv >= 0 && (pv >> 5) == cj) exm[g] |= 1u << (pv & 31);
+          }
         }
       }
+      TC fj[4][3];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) fj[g][0] = fj[g][1] = fj[g][2] = 0;
+      if (info & INFO_MINIMG) {
+        tile_steps<TC, true, true>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, e_c, e_lj);
+      } else {
+        const TC sx = ta.L[0] * (TC)((int)((info >> 16) & 3u) - 1);
+        const TC sy = ta.L[1] * (TC)((int)((info >> 18) & 3u) - 1);
+        const TC sz = ta.L[2] * (TC)((int)((info >> 20) & 3u) - 1);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          pj[g].x += sx;
+          pj[g].y += sy;
+          pj[g].z += sz;
+        }
+        if (info & INFO_EXCL)
+          tile_steps<TC, false, true>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, e_c, e_lj);
+        else
+          tile_steps<TC, false, false>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, e_c, e_lj);
+      }
+      TC v0, v1;
+      reduce_pick2(fj, ii, [](TC v) { return octet_sum(v); }, v0, v1);
+      unsigned long long* base = facc + (long long)cj * (3 * CL);
+      if (v0 != (TC)0) atomicAdd(base + oj0, (unsigned long long)to_fx((double)v0, F_SCALE));
+      if (ii < 4 && v1 != (TC)0) atomicAdd(base + oj1, (unsigned long long)to_fx((double)v1, F_SCALE));
     }
+    if (cur_i >= 0) flush_i(cur_i);
   }
-  if (cur_i >= 0) flush_i(cur_i);
-  // energies: wave sums in double, then fixed point
-  double el = tff_wave_sum((double)e_lj), ecl = tff_wave_sum((double)e_c);
+  // the wave's energies go to one of E_SLOTS fixed-point slots per term (spread to keep the
+  // atomics uncontended; integer sums do not depend on their order)
+  const double el = wave_sum_t((double)e_lj), ecl = wave_sum_t((double)e_c);
   if (lane == 0 && t1 > t0) {
-    atomicAdd(&eacc[0], (unsigned long long)to_fx(el, F_SCALE));
-    atomicAdd(&eacc[1], (unsigned long long)to_fx(ecl, F_SCALE));
+    atomicAdd(&eacc[gw % E_SLOTS], (unsigned long long)to_fx(el, F_SCALE));
+    atomicAdd(&eacc[E_SLOTS + gw % E_SLOTS], (unsigned long long)to_fx(ecl, F_SCALE));
   }
+}
+
+// float: registers capped for 4 waves per SIMD; float64 keeps its full register set
+__global__ __launch_bounds__(TFF_WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_tiles_f32(
+    const int4* __restrict__ cpairs, const int* __restrict__ ncp_ptr, long long cap, const V4<float>* __restrict__ xyzq,
+    const V2<float>* __restrict__ lj, const int* __restrict__ excl_part, TileArgs<float> ta,
+    unsigned long long* __restrict__ facc, unsigned long long* __restrict__ eacc) {
+  tiles_body<float>(cpairs, ncp_ptr, cap, xyzq, lj, excl_part, ta, facc, eacc);
+}
+__global__ __launch_bounds__(TFF_WAVE) void k_tiles_f64(const int4* __restrict__ cpairs,
+                                                       const int* __restrict__ ncp_ptr, long long cap,
+                                                       const V4<double>* __restrict__ xyzq,
+                                                       const V2<double>* __restrict__ lj,
+                                                       const int* __restrict__ excl_part, TileArgs<double> ta,
+                                                       unsigned long long* __restrict__ facc,
+                                                       unsigned long long* __restrict__ eacc) {
+  tiles_body<double>(cpairs, ncp_ptr, cap, xyzq, lj, excl_part, ta, facc, eacc);
 }
 
 // ---------------------------------------------------------------- 5. PME on cluster slots
@@ -700,6 +887,14 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tiles(const int2* __restrict__ c
 // point is written once, so the grid needs neither zeroing nor global atomics.
 constexpr int BR = 8;
 constexpr int BRICK_MAXC = 1024;
+constexpr int SPREAD_CHUNK = 32;  // candidate clusters compacted per round
+
+// Charge-grid fixed point: float mode scales by 2^28 and converts through int32 (a spline-weighted
+// charge |q M M M| < 8 e, i.e. |q| < ~36 e for order >= 4, fits); float64 mode scales by 2^40.
+constexpr double QF_SCALE = 268435456.0;
+__device__ __forceinline__ long long grid_fx(float v) { return (long long)__float2int_rn(v * (float)QF_SCALE); }
+__device__ __forceinline__ long long grid_fx(double v) { return to_fx(v, Q_SCALE); }
+template <typename TC> constexpr double grid_scale() { return sizeof(TC) == 4 ? QF_SCALE : Q_SCALE; }
 
 __device__ __forceinline__ bool circ_overlap(int lo, int len, int b0, int b, int K) {
   if (len >= K) return true;
@@ -726,7 +921,8 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
                                                        unsigned long long* __restrict__ grid, int* __restrict__ status) {
   __shared__ unsigned long long s_q[BR * BR * BR];
   __shared__ int s_list[BRICK_MAXC];
-  __shared__ int s_n;
+  __shared__ int s_atoms[SPREAD_CHUNK * CL];
+  __shared__ int s_n, s_na;
   const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
   const int nbx = (K0 + BR - 1) / BR, nby = (K1 + BR - 1) / BR, nbz = (K2 + BR - 1) / BR;
   const int bid = blockIdx.x;
@@ -777,56 +973,79 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
     if (threadIdx.x == 0) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
     ncand = BRICK_MAXC;
   }
-  // two clusters per wave per round: lane >> 5 picks the cluster, lane & 31 the slot
-  const int lane = threadIdx.x & (TFF_WAVE - 1);
-  const int gw = threadIdx.x / TFF_WAVE, nwv = blockDim.x / TFF_WAVE;
-  for (int r = 2 * gw + (lane >> 5); r < ncand; r += 2 * nwv) {
-    const int sl = s_list[r] * CL + (lane & (CL - 1));
-    if (slot_orig[sl] < 0) continue;
-    const V4<TC> p = xyzq[sl];
-    TC Mv[3][P], dM[3][P];
-    int base[3];
-    unsigned int inmask[3];
-    const double rr[3] = {(double)p.x, (double)p.y, (double)p.z};
+  // Candidate clusters in chunks of SPREAD_CHUNK: first the slots whose stencil reaches the
+  // brick are compacted into s_atoms (a cheap anchor test), then one lane per compacted slot
+  // adds its P^3 contributions to the brick (LDS int64 atomics: the sums do not depend on the
+  // order of the list).
+  for (int r0 = 0; r0 < ncand; r0 += SPREAD_CHUNK) {
+    if (threadIdx.x == 0) s_na = 0;
+    __syncthreads();
+    const int nslot = (ncand - r0 < SPREAD_CHUNK ? ncand - r0 : SPREAD_CHUNK) * CL;
+    for (int k = threadIdx.x; k < nslot; k += blockDim.x) {
+      const int sl = s_list[r0 + k / CL] * CL + (k % CL);
+      if (slot_orig[sl] < 0) continue;
+      const V4<TC> p = xyzq[sl];
+      const double rr[3] = {(double)p.x, (double)p.y, (double)p.z};
+      bool ov = true;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double sa = (rr[0] * pg.inv[a] + rr[1] * pg.inv[3 + a] + rr[2] * pg.inv[6 + a]) * (double)pg.K[a];
-      double t;
-      anchor_axis(sa, pg.K[a], P, base[a], t);
-      spline_weights<TC, P>((TC)t, Mv[a], dM[a]);
-      unsigned int m = 0;
-#pragma unroll
-      for (int k = 0; k < P; ++k) {
-        int gk = base[a] + k;
-        gk = gk >= pg.K[a] ? gk - pg.K[a] : gk;
-        if (gk >= B0[a] && gk < B0[a] + Bn[a]) m |= 1u << k;
+      for (int a = 0; a < 3; ++a) {
+        const double sa = (rr[0] * pg.inv[a] + rr[1] * pg.inv[3 + a] + rr[2] * pg.inv[6 + a]) * (double)pg.K[a];
+        int base;
+        double t;
+        anchor_axis(sa, pg.K[a], P, base, t);
+        ov = ov && circ_overlap(base, P, B0[a], Bn[a], pg.K[a]);
       }
-      inmask[a] = m;
+      if (ov) s_atoms[atomicAdd(&s_na, 1)] = sl;
     }
-    if (!inmask[0] || !inmask[1] || !inmask[2]) continue;
-    const TC qi = p.w;
+    __syncthreads();
+    const int na = s_na;
+    for (int k = threadIdx.x; k < na; k += blockDim.x) {
+      const int sl = s_atoms[k];
+      const V4<TC> p = xyzq[sl];
+      TC Mv[3][P], dM[3][P];
+      int base[3];
+      unsigned int inmask[3];
+      const double rr[3] = {(double)p.x, (double)p.y, (double)p.z};
 #pragma unroll
-    for (int ka = 0; ka < P; ++ka) {
-      if (!((inmask[0] >> ka) & 1u)) continue;
-      int gx = base[0] + ka;
-      gx = gx >= K0 ? gx - K0 : gx;
-      const TC wa = qi * Mv[0][ka];
+      for (int a = 0; a < 3; ++a) {
+        const double sa = (rr[0] * pg.inv[a] + rr[1] * pg.inv[3 + a] + rr[2] * pg.inv[6 + a]) * (double)pg.K[a];
+        double t;
+        anchor_axis(sa, pg.K[a], P, base[a], t);
+        spline_weights<TC, P>((TC)t, Mv[a], dM[a]);
+        unsigned int m = 0;
 #pragma unroll
-      for (int kb = 0; kb < P; ++kb) {
-        if (!((inmask[1] >> kb) & 1u)) continue;
-        int gy = base[1] + kb;
-        gy = gy >= K1 ? gy - K1 : gy;
-        const TC wab = wa * Mv[1][kb];
-        unsigned long long* row = s_q + ((gx - B0[0]) * BR + (gy - B0[1])) * BR;
+        for (int q = 0; q < P; ++q) {
+          int gk = base[a] + q;
+          gk = gk >= pg.K[a] ? gk - pg.K[a] : gk;
+          if (gk >= B0[a] && gk < B0[a] + Bn[a]) m |= 1u << q;
+        }
+        inmask[a] = m;
+      }
+      const TC qi = p.w;
 #pragma unroll
-        for (int kc = 0; kc < P; ++kc) {
-          if (!((inmask[2] >> kc) & 1u)) continue;
-          int gz = base[2] + kc;
-          gz = gz >= K2 ? gz - K2 : gz;
-          atomicAdd(&row[gz - B0[2]], (unsigned long long)to_fx((double)(wab * Mv[2][kc]), Q_SCALE));
+      for (int ka = 0; ka < P; ++ka) {
+        if (!((inmask[0] >> ka) & 1u)) continue;
+        int gx = base[0] + ka;
+        gx = gx >= K0 ? gx - K0 : gx;
+        const TC wa = qi * Mv[0][ka];
+#pragma unroll
+        for (int kb = 0; kb < P; ++kb) {
+          if (!((inmask[1] >> kb) & 1u)) continue;
+          int gy = base[1] + kb;
+          gy = gy >= K1 ? gy - K1 : gy;
+          const TC wab = wa * Mv[1][kb];
+          unsigned long long* row = s_q + ((gx - B0[0]) * BR + (gy - B0[1])) * BR;
+#pragma unroll
+          for (int kc = 0; kc < P; ++kc) {
+            if (!((inmask[2] >> kc) & 1u)) continue;
+            int gz = base[2] + kc;
+            gz = gz >= K2 ? gz - K2 : gz;
+            atomicAdd(&row[gz - B0[2]], (unsigned long long)grid_fx(wab * Mv[2][kc]));
+          }
         }
       }
     }
+    __syncthreads();
   }
   __syncthreads();
   for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) {
@@ -895,54 +1114,40 @@ __global__ __launch_bounds__(FB) void k_gather_slots(const int* __restrict__ ncl
 }
 
 // ---------------------------------------------------------------- 6. finalize
+// Forces back in the caller's atom order; block 0 also sums the energy terms:
+// out = [Lennard-Jones, real-space Coulomb, reciprocal, self, total].
 template <typename TI>
-__global__ __launch_bounds__(FB) void k_forces_out(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
-                                                   const unsigned long long* __restrict__ facc, TI* __restrict__ forces) {
-  int sl = blockIdx.x * FB + threadIdx.x;
-  if (sl >= *ncl_ptr * CL) return;
-  int i = slot_orig[sl];
-  if (i < 0) return;
-  for (int a = 0; a < 3; ++a) forces[3 * i + a] = (TI)((double)(long long)facc[3 * sl + a] / F_SCALE);
-}
-
-template <typename TI>
-__global__ void k_self_q2(const TI* __restrict__ q, long long n, double* __restrict__ part) {
-  double acc = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double v = (double)q[i];
-    acc += v * v;
+__global__ __launch_bounds__(FB) void k_finalize(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
+                                                 const unsigned long long* __restrict__ facc, TI* __restrict__ forces,
+                                                 const unsigned long long* __restrict__ eacc,
+                                                 const double* __restrict__ kparts, int nk,
+                                                 const double* __restrict__ qparts, int nq, double coul,
+                                                 double alpha, double* __restrict__ out) {
+  const int sl = blockIdx.x * FB + threadIdx.x;
+  if (sl < *ncl_ptr * CL) {
+    const int i = slot_orig[sl];
+    if (i >= 0)
+      for (int a = 0; a < 3; ++a) forces[3 * i + a] = (TI)((double)(long long)facc[3 * sl + a] / F_SCALE);
   }
-  double t = tff_block_sum(acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(256) void k_energies_out(const unsigned long long* __restrict__ eacc,
-                                                      const double* __restrict__ kparts, int nk,
-                                                      const double* __restrict__ qparts, int nq, double coul,
-                                                      double alpha, double* __restrict__ out) {
-  double ek = tff_sum_partials(kparts, nk);
-  double q2 = tff_sum_partials(qparts, nq);
+  if (blockIdx.x != 0) return;
+  const double ek = tff_sum_partials(kparts, nk);
+  const double q2 = tff_sum_partials(qparts, nq);
   if (threadIdx.x != 0) return;
-  out[0] = (double)(long long)eacc[0] / F_SCALE;
-  out[1] = (double)(long long)eacc[1] / F_SCALE;
+  long long sl_lj = 0, sl_c = 0;
+  for (int k = 0; k < E_SLOTS; ++k) {
+    sl_lj += (long long)eacc[k];
+    sl_c += (long long)eacc[E_SLOTS + k];
+  }
+  out[0] = (double)sl_lj / F_SCALE;
+  out[1] = (double)sl_c / F_SCALE;
   out[2] = coul * ek;
   out[3] = -coul * alpha * 0.5641895835477563 * q2;
+  out[4] = out[0] + out[1] + out[2] + out[3];
 }
 
-__global__ void k_zero_u64(unsigned long long* p, long long n) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  long long s = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += s) p[i] = 0ull;
-}
 
-__global__ void k_zero_i32(int* p, long long n) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  long long s = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += s) p[i] = 0;
-}
 
 // ---------------------------------------------------------------- host side
-constexpr int QBLOCKS = 64;
 
 bool make_geo(const TffBox& box, long long n, double list_cutoff, Geo& g) {
   const double* b = box.b;
@@ -964,12 +1169,6 @@ bool make_geo(const TffBox& box, long long n, double list_cutoff, Geo& g) {
     // clusters |d| columns apart are at least (|d| - 1) column widths apart
     g.reach[k] = (int)floor(list_cutoff / g.colw[k]) + 1;
   }
-  int nzb = (int)floor(g.L[2] / (a / 8.0));
-  if (nzb < 1) nzb = 1;
-  if (nzb > 4096) nzb = 4096;
-  g.nzb = nzb;
-  g.zbw = g.L[2] / nzb;
-  g.nkeys = g.ncol[0] * g.ncol[1] * g.nzb;
   g.ncl_max = (int)((n + CL - 1) / CL) + g.ncol[0] * g.ncol[1];
   // cluster pairs per i-cluster (half list): clusters whose cube of edge a lies within the
   // dilated cube of edge a + 2 rlist, with a 2.5x margin for cluster shapes, in multiples of 64
@@ -982,17 +1181,27 @@ bool make_geo(const TffBox& box, long long n, double list_cutoff, Geo& g) {
   return true;
 }
 
+// waves of the pair kernel: about 16 tile-list slots each (some 6 listed tiles at water density)
+int tile_waves(const Geo& g) {
+  long long w = (g.pair_cap + 15) / 16;
+  if (w < 256) w = 256;
+  if (w > (1 << 20)) w = 1 << 20;
+  return (int)w;
+}
+
 struct FusedWs {
-  int *key_of, *key_count, *key_start, *cursor, *sorted_orig, *pos_of, *col_cl_start, *ncl, *cl_col, *slot_orig,
-      *excl_part, *ncp, *seg, *seg_count, *seg_start, *slot_of_orig;
+  int *col_of, *off_in_col, *col_count, *col_start, *sorted_orig, *col_cl_start, *ncl, *cl_col, *slot_orig,
+      *excl_part, *ncp, *seg_count, *seg_start, *slot_of_orig;
+  int2* seg;
   void *xyzq, *lj;
   float4 *bb_c, *bb_h, *sb_c, *sb_h;
-  int2* cpairs;
-  unsigned long long *facc, *eacc, *grid;
+  int4* cpairs;
+  unsigned long long *facc, *grid;
   void* work;
   double *kparts, *qparts;
+  unsigned long long* eacc;
   size_t bytes;
-  int nk;
+  int nk, nw_tiles, nq;
 };
 
 FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) {
@@ -1008,12 +1217,11 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   size_t slots = (size_t)g.ncl_max * CL;
   size_t ng = (size_t)K[0] * K[1] * K[2];
   int ncols = g.ncol[0] * g.ncol[1];
-  w.key_of = (int*)take(4 * nn);
-  w.key_count = (int*)take(4 * (size_t)(g.nkeys + 1));
-  w.key_start = (int*)take(4 * (size_t)(g.nkeys + 1));
-  w.cursor = (int*)take(4 * (size_t)(g.nkeys + 1));
+  w.col_of = (int*)take(4 * nn);
+  w.off_in_col = (int*)take(4 * nn);
+  w.col_count = (int*)take(4 * (size_t)(ncols + 1));
+  w.col_start = (int*)take(4 * (size_t)(ncols + 1));
   w.sorted_orig = (int*)take(4 * nn);
-  w.pos_of = (int*)take(4 * nn);
   w.col_cl_start = (int*)take(4 * (size_t)(ncols + 1));
   w.ncl = (int*)take(16);
   w.ncp = (int*)take(16);
@@ -1027,18 +1235,20 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   w.sb_c = (float4*)take(16 * 4 * (size_t)g.ncl_max);
   w.sb_h = (float4*)take(16 * 4 * (size_t)g.ncl_max);
   w.slot_of_orig = (int*)take(4 * nn);
-  w.cpairs = (int2*)take(8 * (size_t)g.pair_cap);
-  w.seg = (int*)take(4 * (size_t)g.pair_cap);
+  w.cpairs = (int4*)take(16 * (size_t)g.pair_cap);
+  w.seg = (int2*)take(8 * (size_t)g.pair_cap);
   w.seg_count = (int*)take(4 * (size_t)g.ncl_max);
   w.seg_start = (int*)take(4 * (size_t)g.ncl_max);
   w.facc = (unsigned long long*)take(8 * 3 * slots);
-  w.eacc = (unsigned long long*)take(8 * 4);
+  w.nw_tiles = tile_waves(g);
+  w.eacc = (unsigned long long*)take(8 * 2 * E_SLOTS);
   w.grid = (unsigned long long*)take(8 * ng);
   w.work = take(2 * tc * ng);
   PassCfg px = pass_cfg(1, K[0], K[1] * K[2], 2 * tc);
   w.nk = px.tiles;
   w.kparts = (double*)take(8 * (size_t)w.nk);
-  w.qparts = (double*)take(8 * QBLOCKS);
+  w.nq = (int)((nn + FB - 1) / FB);
+  w.qparts = (double*)take(8 * (size_t)w.nq);
   w.bytes = off;
   return w;
 }
@@ -1077,45 +1287,38 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   const int nslots = g.ncl_max * CL;
   const unsigned nbS = (unsigned)((nslots + FB - 1) / FB);
   const int ncols = g.ncol[0] * g.ncol[1];
-  auto zero32 = [&](int* p, long long m) {
-    long long b = (m + 255) / 256;
-    if (b > 4096) b = 4096;
-    if (b < 1) b = 1;
-    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)b), dim3(256), 0, st, p, m);
-  };
-  auto zero64 = [&](unsigned long long* p, long long m) {
-    long long b = (m + 255) / 256;
-    if (b > 8192) b = 8192;
-    if (b < 1) b = 1;
-    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)b), dim3(256), 0, st, p, m);
-  };
-  zero32(w.key_count, g.nkeys + 1);
-  zero32(w.cursor, g.nkeys + 1);
-  zero32(w.ncp, 4);
-  zero32(status, 2);
-  zero64(w.facc, 3LL * nslots);
-  zero64(w.eacc, 4);
+  {
+    ClearList cl{};
+    auto add = [&](void* p, long long words) {
+      cl.p[cl.count] = (int*)p;
+      cl.n[cl.count] = words;
+      ++cl.count;
+    };
+    add(w.col_count, ncols + 1);
+    add(w.ncp, 4);
+    add(status, 2);
+    add(w.facc, 6LL * nslots);
+    add(w.slot_orig, nslots);
+    add(w.eacc, 4 * E_SLOTS);
+    hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, cl);
+  }
   // 1. sort
-  hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, n, g, w.key_of, w.key_count);
-  hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(1024), 0, st, w.key_count, g.nkeys, w.key_start);
-  hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.key_start, ncols, g.nzb, w.col_cl_start, w.ncl);
-  hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.key_of, w.key_start, w.cursor,
-                     w.sorted_orig, w.pos_of);
-  hipLaunchKernelGGL(k_sort_keys, dim3((g.nkeys + FB - 1) / FB), dim3(FB), 0, st, g.nkeys, w.key_start,
-                     w.sorted_orig, w.pos_of);
+  hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, q, n, g, w.col_of, w.off_in_col,
+                     w.col_count, w.qparts);
+  hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.col_count, ncols, w.col_start, w.col_cl_start,
+                     w.ncl, status);
+  hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
+                     w.sorted_orig);
+  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(256), 0, st, coords, g, w.col_start, w.sorted_orig);
   // 2. cluster arrays
-  zero32(w.slot_orig, nslots);
   hipLaunchKernelGGL((k_fill_clusters<TI, TC>), dim3(nbS), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
-                     w.col_cl_start, w.key_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
+                     w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
                      w.cl_col);
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
   hipLaunchKernelGGL(k_reorder_clusters<TC>, dim3(nbW), dim3(FB), 0, st, w.ncl, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj,
-                     w.slot_orig, w.slot_of_orig);
-  hipLaunchKernelGGL(k_clear_pad_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.excl_part);
-  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbA), dim3(FB), 0, st, n, w.slot_of_orig, row_ptr, col, w.excl_part,
-                     status);
-  hipLaunchKernelGGL(k_bbox<TC>, dim3(nbW), dim3(FB), 0, st, w.ncl, (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c,
-                     w.bb_h, w.sb_c, w.sb_h);
+                     w.slot_orig, w.slot_of_orig, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
+  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
+                     col, w.excl_part, status);
   // 3. cluster pairs
   {
     void* tok = tff_prof_begin("fused_cluster_pairs", st);
@@ -1123,8 +1326,10 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c,
                        w.bb_h, g, rl2, w.excl_part, w.seg, w.seg_count, status);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
+    // sub-tiles are pruned at the interaction cutoff itself: pairs between rc and the list
+    // cutoff contribute nothing to this evaluation
     hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
-                       w.cpairs);
+                       w.sb_c, w.sb_h, g, (float)(cutoff * cutoff), w.cpairs);
     tff_prof_end(tok, st);
   }
   // 4. tiles
@@ -1137,13 +1342,15 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     ta.rc2 = (TC)(cutoff * cutoff);
     ta.alpha = (TC)alpha;
     ta.coul = (TC)coul;
-    long long want = (g.pair_cap + TILE_WAVES - 1) / TILE_WAVES;
-    long long blocks = want < 4096 ? want : 4096;
-    if (blocks < 1) blocks = 1;
     void* tok = tff_prof_begin("fused_tiles", st);
-    hipLaunchKernelGGL(k_tiles<TC>, dim3((unsigned)blocks), dim3(TILE_BLOCK), 0, st, w.cpairs, w.ncp, g.pair_cap,
-                       (const V4<TC>*)w.xyzq, (const V2<TC>*)w.lj, w.slot_orig, w.excl_part, w.sb_c, w.sb_h, ta,
-                       (float)(list_cutoff * list_cutoff), w.facc, w.eacc);
+    if constexpr (sizeof(TC) == 4)
+      hipLaunchKernelGGL(k_tiles_f32, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
+                         g.pair_cap, (const V4<float>*)w.xyzq, (const V2<float>*)w.lj, w.excl_part, ta, w.facc,
+                         w.eacc);
+    else
+      hipLaunchKernelGGL(k_tiles_f64, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
+                         g.pair_cap, (const V4<double>*)w.xyzq, (const V2<double>*)w.lj, w.excl_part, ta, w.facc,
+                         w.eacc);
     tff_prof_end(tok, st);
   }
   // 5. PME
@@ -1162,7 +1369,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   cv.vol = box.det;
   cv.alpha = alpha;
   cv.grid_fixed = 1;
-  cv.grid_inv_scale = 1.0 / Q_SCALE;
+  cv.grid_inv_scale = 1.0 / grid_scale<TC>();
   const size_t ce = 2 * sizeof(TC);
   PassCfg cz = pass_cfg(K[0] * K[1], K[2], 1, ce);
   PassCfg cy = pass_cfg(K[0], K[1], K[2], ce);
@@ -1189,10 +1396,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     if (rc) return rc;
   }
   // 6. outputs
-  hipLaunchKernelGGL(k_forces_out<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, forces);
-  hipLaunchKernelGGL(k_self_q2<TI>, dim3(QBLOCKS), dim3(256), 0, st, q, n, w.qparts);
-  hipLaunchKernelGGL(k_energies_out, dim3(1), dim3(256), 0, st, w.eacc, w.kparts, w.nk, w.qparts, QBLOCKS, coul,
-                     alpha, energies);
+  hipLaunchKernelGGL(k_finalize<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, forces, w.eacc,
+                     w.kparts, w.nk, w.qparts, w.nq, coul, alpha, energies);
   TFF_CHECK_LAUNCH("fused energy+forces");
   return TFF_OK;
 }

@@ -25,6 +25,7 @@ from .workloads import COULOMB_KJMOL
 
 PAIR_OVERFLOW = 1
 EXCL_OVERFLOW = 2
+COLUMN_OVERFLOW = 4
 
 
 class FusedNonbondedPME(nn.Module):
@@ -93,23 +94,23 @@ class FusedNonbondedPME(nn.Module):
             self.excl_row_ptr = self.excl_row_ptr.to(dev)
             self.excl_col_indices = self.excl_col_indices.to(dev)
         rp, col = self.excl_row_ptr, self.excl_col_indices
-        terms = torch.empty(4, dtype=torch.float64, device=dev)
+        out = torch.empty(5, dtype=torch.float64, device=dev)
         forces = torch.empty_like(c)
         status = torch.empty(2, dtype=torch.int32, device=dev)
         cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
         rc = lib.tff_fused_energy_forces(L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms,
                                          bh, L.ptr(rp), L.ptr(col), self.cutoff, self.list_cutoff, self.alpha,
                                          self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
-                                         L.ptr(terms), L.ptr(forces), L.ptr(status), L.ptr(ws), ws.numel(),
+                                         L.ptr(out), L.ptr(forces), L.ptr(status), L.ptr(ws), ws.numel(),
                                          L.stream_of(c))
         L.check(rc, "FusedNonbondedPME")
         self.status = status
         if check:
             self.check()
-        return terms.sum(), forces, terms
+        return out[4], forces, out[:4]
 
     def tiles(self):
-        """Number of 32 x 32 cluster-pair tiles of the last call (synchronises)."""
+        """Number of 32 x 32 cluster-pair tiles within the list cutoff in the last call (synchronises)."""
         return None if self.status is None else int(self.status[1].item())
 
     def check(self):
@@ -121,6 +122,9 @@ class FusedNonbondedPME(nn.Module):
             raise RuntimeError("FusedNonbondedPME: cluster-pair capacity exceeded")
         if st & EXCL_OVERFLOW:
             raise RuntimeError("FusedNonbondedPME: an atom has more than 8 exclusion partners")
+        if st & COLUMN_OVERFLOW:
+            raise RuntimeError("FusedNonbondedPME: an x-y column holds more than 8192 atoms (strongly "
+                               "inhomogeneous system); results are not reproducible bit for bit")
 
 
 def for_workload(w, precision="mixed"):
