@@ -888,6 +888,7 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_f64(const int4* __restrict__
 constexpr int BR = 8;
 constexpr int BRICK_MAXC = 1024;
 constexpr int SPREAD_CHUNK = 32;  // candidate clusters compacted per round
+constexpr int SPREAD_MAXCOL = 64; // candidate columns scanned per round
 
 // Charge-grid fixed point: float mode scales by 2^28 and converts through int32 (a spline-weighted
 // charge |q M M M| < 8 e, i.e. |q| < ~36 e for order >= 4, fits); float64 mode scales by 2^40.
@@ -921,7 +922,8 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
                                                        unsigned long long* __restrict__ grid, int* __restrict__ status) {
   __shared__ unsigned long long s_q[BR * BR * BR];
   __shared__ int s_list[BRICK_MAXC];
-  __shared__ int s_atoms[SPREAD_CHUNK * CL];
+  __shared__ V4<TC> s_pos[SPREAD_CHUNK * CL];
+  __shared__ int s_coff[SPREAD_MAXCOL + 1], s_ccs[SPREAD_MAXCOL];
   __shared__ int s_n, s_na;
   const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
   const int nbx = (K0 + BR - 1) / BR, nby = (K1 + BR - 1) / BR, nbz = (K2 + BR - 1) / BR;
@@ -934,7 +936,7 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
   for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) s_q[k] = 0ull;
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
-  // candidate clusters: columns whose x-y extent can reach the brick
+  // candidate columns: those whose x-y extent can reach the brick
   const float kl[3] = {(float)(pg.K[0] * g.invL[0]), (float)(pg.K[1] * g.invL[1]), (float)(pg.K[2] * g.invL[2])};
   int c0[2], nc[2];
   for (int a = 0; a < 2; ++a) {
@@ -945,38 +947,58 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
     nc[a] = ch - cl + 1 < g.ncol[a] ? ch - cl + 1 : g.ncol[a];
   }
   const int ncand_cols = nc[0] * nc[1];
-  for (int cc = 0; cc < ncand_cols; ++cc) {
-    int xx = (c0[0] + cc / nc[1]) % g.ncol[0];
-    if (xx < 0) xx += g.ncol[0];
-    int yy = (c0[1] + cc % nc[1]) % g.ncol[1];
-    if (yy < 0) yy += g.ncol[1];
-    const int col = xx * g.ncol[1] + yy;
-    const int cs = col_cl_start[col], ce = col_cl_start[col + 1];
-    for (int c = cs + (int)threadIdx.x; c < ce; c += blockDim.x) {
+  // the candidate columns' cluster ranges, scanned into one flat range (columns in groups of
+  // SPREAD_MAXCOL), then every candidate cluster tested in one pass
+  for (int g0 = 0; g0 < ncand_cols; g0 += SPREAD_MAXCOL) {
+    const int ngc = ncand_cols - g0 < SPREAD_MAXCOL ? ncand_cols - g0 : SPREAD_MAXCOL;
+    if (threadIdx.x == 0) {
+      int run = 0;
+      for (int t = 0; t < ngc; ++t) {
+        const int cc = g0 + t;
+        int xx = (c0[0] + cc / nc[1]) % g.ncol[0];
+        if (xx < 0) xx += g.ncol[0];
+        int yy = (c0[1] + cc % nc[1]) % g.ncol[1];
+        if (yy < 0) yy += g.ncol[1];
+        const int col = xx * g.ncol[1] + yy;
+        const int cs = col_cl_start[col], ce = col_cl_start[col + 1];
+        s_coff[t] = run;
+        s_ccs[t] = cs;
+        run += ce - cs;
+      }
+      s_coff[ngc] = run;
+    }
+    __syncthreads();
+    const int total = s_coff[ngc];
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+      int lo = 0, hi = ngc - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_coff[mid] <= k) lo = mid; else hi = mid - 1;
+      }
+      const int c = s_ccs[lo] + (k - s_coff[lo]);
       const float4 cc4 = bb_c[c], hh4 = bb_h[c];
       const float cv[3] = {cc4.x, cc4.y, cc4.z}, hv[3] = {hh4.x, hh4.y, hh4.z};
       bool ok = true;
       for (int a = 0; a < 3; ++a) {
-        int lo, len;
-        stencil_interval((cv[a] - hv[a]) * kl[a], (cv[a] + hv[a]) * kl[a], P, lo, len);
-        ok = ok && circ_overlap(lo, len, B0[a], Bn[a], pg.K[a]);
+        int lo2, len;
+        stencil_interval((cv[a] - hv[a]) * kl[a], (cv[a] + hv[a]) * kl[a], P, lo2, len);
+        ok = ok && circ_overlap(lo2, len, B0[a], Bn[a], pg.K[a]);
       }
       if (ok) {
-        int k = atomicAdd(&s_n, 1);
-        if (k < BRICK_MAXC) s_list[k] = c;
+        const int q = atomicAdd(&s_n, 1);
+        if (q < BRICK_MAXC) s_list[q] = c;
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   int ncand = s_n;
   if (ncand > BRICK_MAXC) {
     if (threadIdx.x == 0) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
     ncand = BRICK_MAXC;
   }
-  // Candidate clusters in chunks of SPREAD_CHUNK: first the slots whose stencil reaches the
-  // brick are compacted into s_atoms (a cheap anchor test), then one lane per compacted slot
-  // adds its P^3 contributions to the brick (LDS int64 atomics: the sums do not depend on the
-  // order of the list).
+  // Candidate clusters in chunks of SPREAD_CHUNK: the slots whose stencil reaches the brick are
+  // compacted into s_pos (a cheap anchor test), then one lane per compacted slot adds its P^3
+  // contributions to the brick (LDS int64 atomics: the sums do not depend on the list order).
   for (int r0 = 0; r0 < ncand; r0 += SPREAD_CHUNK) {
     if (threadIdx.x == 0) s_na = 0;
     __syncthreads();
@@ -995,13 +1017,12 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
         anchor_axis(sa, pg.K[a], P, base, t);
         ov = ov && circ_overlap(base, P, B0[a], Bn[a], pg.K[a]);
       }
-      if (ov) s_atoms[atomicAdd(&s_na, 1)] = sl;
+      if (ov) s_pos[atomicAdd(&s_na, 1)] = p;
     }
     __syncthreads();
     const int na = s_na;
     for (int k = threadIdx.x; k < na; k += blockDim.x) {
-      const int sl = s_atoms[k];
-      const V4<TC> p = xyzq[sl];
+      const V4<TC> p = s_pos[k];
       TC Mv[3][P], dM[3][P];
       int base[3];
       unsigned int inmask[3];

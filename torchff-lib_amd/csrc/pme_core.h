@@ -128,19 +128,27 @@ __device__ __forceinline__ void butterfly(cx<T>* v) {
   }
 }
 
+// x / d and x % d for 0 <= x < 2^22, 1 <= d <= 4096 through a float reciprocal: (x + 0.5) / d is at
+// least 0.5 / d from an integer and the float error is below 2^-22 (x + 0.5) / d, so the
+// truncation is exact.
+__device__ __forceinline__ int fdiv(int x, float inv_d) { return (int)(((float)x + 0.5f) * inv_d); }
+
 // One Stockham stage of radix R over nl lines of length K (line stride LS) held in LDS.
+// nl is a power of two (nl = 1 << nl_shift); work item w -> line w & (nl - 1), index w >> nl_shift.
 template <typename T, int DIR, int R>
 __device__ __forceinline__ void fft_stage(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst,
-                                          const cx<T>* __restrict__ tw, int K, int LS, int nl, int Ns) {
+                                          const cx<T>* __restrict__ tw, int K, int LS, int nl_shift, int Ns) {
   const int KR = K / R;
-  const int items = nl * KR;
+  const int items = KR << nl_shift;
   const int tstep = K / (Ns * R);
+  const float inv_ns = 1.0f / (float)Ns;
   for (int w = threadIdx.x; w < items; w += blockDim.x) {
-    const int line = w / KR;
-    const int j = w - line * KR;
+    const int line = w & ((1 << nl_shift) - 1);
+    const int j = w >> nl_shift;
     const cx<T>* in = src + line * LS;
     cx<T>* o = dst + line * LS;
-    const int jm = j % Ns;
+    const int jq = fdiv(j, inv_ns);
+    const int jm = j - jq * Ns;
     cx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -153,7 +161,7 @@ __device__ __forceinline__ void fft_stage(const cx<T>* __restrict__ src, cx<T>* 
       v[r] = x;
     }
     butterfly<T, DIR, R>(v);
-    const int idxD = (j / Ns) * Ns * R + jm;
+    const int idxD = jq * Ns * R + jm;
 #pragma unroll
     for (int r = 0; r < R; ++r) o[idxD + r * Ns] = v[r];
   }
@@ -162,20 +170,20 @@ __device__ __forceinline__ void fft_stage(const cx<T>* __restrict__ src, cx<T>* 
 // Stockham transform of nl lines of length K (line stride LS) held in LDS.
 // Returns the buffer holding the result (a or b).
 template <typename T, int DIR>
-__device__ cx<T>* fft_lines(cx<T>* a, cx<T>* b, const cx<T>* tw, int K, int LS, int nl, const FftPlan& plan) {
+__device__ cx<T>* fft_lines(cx<T>* a, cx<T>* b, const cx<T>* tw, int K, int LS, int nl_shift, const FftPlan& plan) {
   cx<T>* src = a;
   cx<T>* dst = b;
   int Ns = 1;
   for (int s = 0; s < plan.nst; ++s) {
     const int R = plan.radix[s];
     if (R == 4)
-      fft_stage<T, DIR, 4>(src, dst, tw, K, LS, nl, Ns);
+      fft_stage<T, DIR, 4>(src, dst, tw, K, LS, nl_shift, Ns);
     else if (R == 2)
-      fft_stage<T, DIR, 2>(src, dst, tw, K, LS, nl, Ns);
+      fft_stage<T, DIR, 2>(src, dst, tw, K, LS, nl_shift, Ns);
     else if (R == 3)
-      fft_stage<T, DIR, 3>(src, dst, tw, K, LS, nl, Ns);
+      fft_stage<T, DIR, 3>(src, dst, tw, K, LS, nl_shift, Ns);
     else
-      fft_stage<T, DIR, 5>(src, dst, tw, K, LS, nl, Ns);
+      fft_stage<T, DIR, 5>(src, dst, tw, K, LS, nl_shift, Ns);
     __syncthreads();
     cx<T>* t = src;
     src = dst;
@@ -189,7 +197,8 @@ enum PassMode { PASS_Z_FWD = 0, PASS_Y_FWD = 1, PASS_X_CONV = 2, PASS_Y_INV = 3,
 
 struct PassArgs {
   int A, K, C;   // element (a, k, c) at (a*K + k)*C + c
-  int AB, CB;    // tile: AB values of a, CB values of c
+  int AB, CB;    // tile: AB values of a, CB values of c (powers of two, one of them 1)
+  int ab_shift, cb_shift;
   int LS;        // LDS line stride
 };
 
@@ -230,12 +239,19 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
   const int a0 = (tile / ntc) * pa.AB;
   const int c0 = (tile % ntc) * pa.CB;
   const int nelem = nl * K;
+  const int nl_shift = pa.ab_shift + pa.cb_shift;
+  const float inv_k = 1.0f / (float)K;
+  // element idx of the tile -> (ab, k, cb): idx = (ab * K + k) * CB + cb
+  auto split = [&](int idx, int& ab, int& k, int& cb) {
+    cb = idx & (pa.CB - 1);
+    const int rest = idx >> pa.cb_shift;
+    ab = pa.AB == 1 ? 0 : fdiv(rest, inv_k);
+    k = rest - ab * K;
+  };
   // load
   for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
-    int cb = idx % pa.CB;
-    int rest = idx / pa.CB;
-    int k = rest % K;
-    int ab = rest / K;
+    int ab, k, cb;
+    split(idx, ab, k, cb);
     int a = a0 + ab, c = c0 + cb;
     cx<T> v = {(T)0, (T)0};
     if (a < pa.A && c < pa.C) {
@@ -254,15 +270,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
   __syncthreads();
   cx<T>* res;
   if (MODE == PASS_Z_FWD || MODE == PASS_Y_FWD || MODE == PASS_X_CONV)
-    res = fft_lines<T, -1>(bufA, bufB, tw, K, LS, nl, plan);
+    res = fft_lines<T, -1>(bufA, bufB, tw, K, LS, nl_shift, plan);
   else
-    res = fft_lines<T, 1>(bufA, bufB, tw, K, LS, nl, plan);
+    res = fft_lines<T, 1>(bufA, bufB, tw, K, LS, nl_shift, plan);
   if (MODE == PASS_X_CONV) {
     // lines are (y, z) = c; along the line k = mx
     double e_acc = 0.0;
     const T two_pi = (T)6.283185307179586;
     for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
-      int line = idx / K;
+      int line = fdiv(idx, inv_k);
       int k = idx - line * K;
       int c = c0 + line;  // AB == 1 for this pass
       if (c >= pa.C) continue;
@@ -285,16 +301,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
     }
     __syncthreads();
     cx<T>* other = (res == bufA) ? bufB : bufA;
-    res = fft_lines<T, 1>(res, other, tw, K, LS, nl, plan);
+    res = fft_lines<T, 1>(res, other, tw, K, LS, nl_shift, plan);
     double tot = tff_block_sum(e_acc);
     if (threadIdx.x == 0) e_part[blockIdx.x] = tot;
   }
   // store
   for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
-    int cb = idx % pa.CB;
-    int rest = idx / pa.CB;
-    int k = rest % K;
-    int ab = rest / K;
+    int ab, k, cb;
+    split(idx, ab, k, cb);
     int a = a0 + ab, c = c0 + cb;
     if (a < pa.A && c < pa.C) {
       long long g = ((long long)a * K + k) * pa.C + c;
@@ -320,6 +334,9 @@ PassCfg pass_cfg(int A, int K, int C, size_t celem) {
   int lines = (int)((FFT_LDS_BUDGET - (size_t)K * celem) / per_line);
   if (lines < 1) lines = 1;
   if (lines > 16) lines = 16;  // more, smaller tiles: >= 512 workgroups for a 96^3 grid
+  int p2 = 1;
+  while (p2 * 2 <= lines) p2 *= 2;  // a power of two: line indices by mask and shift
+  lines = p2;
   c.pa.A = A;
   c.pa.K = K;
   c.pa.C = C;
@@ -331,6 +348,13 @@ PassCfg pass_cfg(int A, int K, int C, size_t celem) {
     c.pa.AB = 1;
     c.pa.CB = lines < C ? lines : C;
   }
+  // AB / CB: the largest power of two not above lines and A / C
+  while (c.pa.AB & (c.pa.AB - 1)) c.pa.AB &= c.pa.AB - 1;
+  while (c.pa.CB & (c.pa.CB - 1)) c.pa.CB &= c.pa.CB - 1;
+  c.pa.ab_shift = 0;
+  while ((1 << c.pa.ab_shift) < c.pa.AB) ++c.pa.ab_shift;
+  c.pa.cb_shift = 0;
+  while ((1 << c.pa.cb_shift) < c.pa.CB) ++c.pa.cb_shift;
   int ntc = (C + c.pa.CB - 1) / c.pa.CB;
   int nta = (A + c.pa.AB - 1) / c.pa.AB;
   c.tiles = ntc * nta;
