@@ -706,12 +706,16 @@ __device__ __forceinline__ void tile_steps(unsigned mask, bool upper, const V4<T
       dy -= ta.L[1] * rint(dy * ta.invL[1]);
       dz -= ta.L[2] * rint(dz * ta.invL[2]);
     }
-    const TC r2 = dx * dx + dy * dy + dz * dz;
+    TC r2 = dx * dx + dy * dy + dz * dz;
     bool ok = r2 <= ta.rc2;
     if (MINIMG) ok = ok && pi[gi].x < (TC)PAD_LIMIT && pj[gj].x < (TC)PAD_LIMIT;
     if (gi == gj) ok = ok && upper;
     bool excl = false;
     if (EXCL) excl = (exm[gi] >> (gj * 8 + jj)) & 1u;
+    // pairs that do not count are moved far away: every term below is then exactly zero
+    // (exp underflows, s^6 flushes), except (erfc - 1)/r of an excluded pair beyond the
+    // cutoff, which is below 1e-12 q_i q_j
+    r2 = ok ? r2 : (TC)1e30;
     const TC rinv = rsqrt_t(r2);
     const TC rinv2 = rinv * rinv;
     const TC u = ta.alpha * (r2 * rinv);
@@ -728,9 +732,8 @@ __device__ __forceinline__ void tile_steps(unsigned mask, bool upper, const V4<T
     const TC s6 = s2 * s2 * s2;
     const TC elj = ep4 * fma(s6, s6, -s6);
     fs = fma(ep4 * s6 * fma((TC)12, s6, (TC)-6), rinv2, fs);
-    fs = ok ? fs : (TC)0;
-    e_c += ok ? ecr : (TC)0;
-    e_lj += ok ? elj : (TC)0;
+    e_c += ecr;
+    e_lj += elj;
     // F_i += fs d, F_j -= fs d   (fs = -dE/dr / r)
     fi[gi][0] = fma(fs, dx, fi[gi][0]);
     fi[gi][1] = fma(fs, dy, fi[gi][1]);
@@ -782,7 +785,8 @@ __device__ __forceinline__ void tiles_body(const int4* __restrict__ cpairs, cons
     if (v0 != (TC)0) atomicAdd(base + oi0, (unsigned long long)to_fx((double)v0, F_SCALE));
     if (jj < 4 && v1 != (TC)0) atomicAdd(base + oi1, (unsigned long long)to_fx((double)v1, F_SCALE));
   };
-  TC e_lj = 0, e_c = 0;
+  // energies: float within a tile (at most 16 terms per lane), then double across tiles
+  double e_lj = 0.0, e_c = 0.0;
   int cur_i = -1;
   {
     for (long long t = t0; t < t1; ++t) {
@@ -828,8 +832,9 @@ __device__ __forceinline__ void tiles_body(const int4* __restrict__ cpairs, cons
       TC fj[4][3];
 #pragma unroll
       for (int g = 0; g < 4; ++g) fj[g][0] = fj[g][1] = fj[g][2] = 0;
+      TC t_lj = 0, t_c = 0;
       if (info & INFO_MINIMG) {
-        tile_steps<TC, true, true>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, e_c, e_lj);
+        tile_steps<TC, true, true>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, t_c, t_lj);
       } else {
         const TC sx = ta.L[0] * (TC)((int)((info >> 16) & 3u) - 1);
         const TC sy = ta.L[1] * (TC)((int)((info >> 18) & 3u) - 1);
@@ -841,10 +846,12 @@ __device__ __forceinline__ void tiles_body(const int4* __restrict__ cpairs, cons
           pj[g].z += sz;
         }
         if (info & INFO_EXCL)
-          tile_steps<TC, false, true>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, e_c, e_lj);
+          tile_steps<TC, false, true>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, t_c, t_lj);
         else
-          tile_steps<TC, false, false>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, e_c, e_lj);
+          tile_steps<TC, false, false>(mask, upper, pi, li, pj, lJ, exm, jj, ta, fi, fj, t_c, t_lj);
       }
+      e_lj += (double)t_lj;
+      e_c += (double)t_c;
       TC v0, v1;
       reduce_pick2(fj, ii, [](TC v) { return octet_sum(v); }, v0, v1);
       unsigned long long* base = facc + (long long)cj * (3 * CL);
@@ -855,7 +862,7 @@ __device__ __forceinline__ void tiles_body(const int4* __restrict__ cpairs, cons
   }
   // the wave's energies go to one of E_SLOTS fixed-point slots per term (spread to keep the
   // atomics uncontended; integer sums do not depend on their order)
-  const double el = wave_sum_t((double)e_lj), ecl = wave_sum_t((double)e_c);
+  const double el = wave_sum_t(e_lj), ecl = wave_sum_t(e_c);
   if (lane == 0 && t1 > t0) {
     atomicAdd(&eacc[gw % E_SLOTS], (unsigned long long)to_fx(el, F_SCALE));
     atomicAdd(&eacc[E_SLOTS + gw % E_SLOTS], (unsigned long long)to_fx(ecl, F_SCALE));
@@ -897,12 +904,22 @@ __device__ __forceinline__ long long grid_fx(float v) { return (long long)__floa
 __device__ __forceinline__ long long grid_fx(double v) { return to_fx(v, Q_SCALE); }
 template <typename TC> constexpr double grid_scale() { return sizeof(TC) == 4 ? QF_SCALE : Q_SCALE; }
 
+// x mod K: two conditional corrections each way cover -2K <= x < 3K (grid indices within a
+// few stencil widths of [0, K)); anything further out (tiny grids) takes the division
+__device__ __forceinline__ int mod_near(int x, int K) {
+  x = x < 0 ? x + K : x;
+  x = x < 0 ? x + K : x;
+  x = x >= K ? x - K : x;
+  x = x >= K ? x - K : x;
+  if (x < 0 || x >= K) x = ((x % K) + K) % K;
+  return x;
+}
+
+// do the circular index intervals [lo, lo + len) and [b0, b0 + b) of Z_K intersect?
 __device__ __forceinline__ bool circ_overlap(int lo, int len, int b0, int b, int K) {
   if (len >= K) return true;
-  int d1 = (b0 - lo) % K;
-  if (d1 < 0) d1 += K;
-  int d2 = (lo - b0) % K;
-  if (d2 < 0) d2 += K;
+  const int d1 = mod_near(b0 - lo, K);
+  const int d2 = mod_near(lo - b0, K);
   return d1 < len || d2 < b;
 }
 

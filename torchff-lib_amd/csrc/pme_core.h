@@ -75,10 +75,14 @@ __device__ __forceinline__ void anchor_axis(double s, int K, int P, int& base, d
   double shift = (P & 1) ? 0.5 : 0.0;
   double m0 = ceil(s - shift);
   t = m0 - s + 0.5 * P - (double)(P / 2);
-  long long b = (long long)m0 - P / 2;
-  b %= K;
-  if (b < 0) b += K;
-  base = (int)b;
+  // (m0 - P / 2) mod K in double: integer-valued operands below 2^52, so the product and the
+  // difference are exact; the quotient may be off by one, fixed by the two corrections
+  const double b = m0 - (double)(P / 2);
+  const double Kd = (double)K;
+  double r = b - Kd * floor(b / Kd);
+  r = r < 0.0 ? r + Kd : r;
+  r = r >= Kd ? r - Kd : r;
+  base = (int)r;
 }
 
 // ------------------------------------------------------------------ FFT
@@ -230,9 +234,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
   cx<T>* bufB = bufA + nl * LS;
   // twiddles exp(-2 pi i m / K)
   for (int m = threadIdx.x; m < K; m += blockDim.x) {
-    double sn, cs;
-    sincospi(-2.0 * (double)m / (double)K, &sn, &cs);
-    tw[m] = {(T)cs, (T)sn};
+    if constexpr (sizeof(T) == 4) {
+      float sn, cs;
+      sincospif(-2.0f * (float)m / (float)K, &sn, &cs);
+      tw[m] = {cs, sn};
+    } else {
+      double sn, cs;
+      sincospi(-2.0 * (double)m / (double)K, &sn, &cs);
+      tw[m] = {(T)cs, (T)sn};
+    }
   }
   const int ntc = (pa.C + pa.CB - 1) / pa.CB;
   const int tile = blockIdx.x;
