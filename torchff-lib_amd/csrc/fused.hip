@@ -190,10 +190,12 @@ __global__ __launch_bounds__(256) void k_sort_columns(const TI* __restrict__ coo
     s_k[k] = key;
   }
   __syncthreads();
+  // pair k -> keys (lo, lo + stride); for stride <= 64 the pairs of one wave (k in a run of 64,
+  // blockDim a multiple of 64) touch only that wave's own 128 keys, so a wave barrier suffices
   for (int size = 2; size <= np; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       for (int k = threadIdx.x; k < (np >> 1); k += blockDim.x) {
-        const int lo = (k / stride) * 2 * stride + (k % stride), hi = lo + stride;
+        const int lo = (k & ~(stride - 1)) * 2 + (k & (stride - 1)), hi = lo + stride;
         const bool asc = (lo & size) == 0;
         const unsigned long long x = s_k[lo], y = s_k[hi];
         if ((x > y) == asc) {
@@ -201,9 +203,17 @@ __global__ __launch_bounds__(256) void k_sort_columns(const TI* __restrict__ coo
           s_k[hi] = x;
         }
       }
-      __syncthreads();
+      const int next = stride > 1 ? stride >> 1 : size;
+      if (stride > TFF_WAVE || next > TFF_WAVE) {
+        __syncthreads();
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
   }
+  __syncthreads();
   for (int k = threadIdx.x; k < m; k += blockDim.x) sorted_orig[cs + k] = (int)(unsigned)(s_k[k] & 0xffffffffull);
 }
 
@@ -220,70 +230,52 @@ __global__ void k_clear(ClearList c) {
 }
 
 // ---------------------------------------------------------------- 2. cluster arrays
-// One thread per slot of the padded cluster arrays (ncl * 32 slots).
+// One wave per cluster (lanes 0..31 = slots): the cluster takes 32 consecutive atoms of its
+// column's z-sorted range (the last cluster of a column is padded).  The slots are then
+// ordered into four spatially compact groups of 8: sort by x and split in halves, sort each
+// half by y and split, order each group by z (padding slots, keyed +inf, collect at the end;
+// ties broken by slot).  Writes the cluster arrays (TC4 x, y, z, q; TC2 sigma / 2,
+// 2 sqrt(eps)), slot_orig / slot_of_orig, the cluster's column, and the bounding boxes (centre,
+// half extent) of the cluster and of its four groups (an empty group gets half extent -1),
+// widened by a relative epsilon so that float rounding of the tests never drops a pair.
 template <typename TI, typename TC>
-__global__ __launch_bounds__(FB) void k_fill_clusters(const TI* __restrict__ coords, const TI* __restrict__ q,
-                                                      const TI* __restrict__ sig, const TI* __restrict__ eps,
-                                                      Geo g, const int* __restrict__ ncl_ptr,
-                                                      const int* __restrict__ col_cl_start,
-                                                      const int* __restrict__ col_start,
-                                                      const int* __restrict__ sorted_orig,
-                                                      V4<TC>* __restrict__ xyzq, V2<TC>* __restrict__ lj,
-                                                      int* __restrict__ slot_orig, int* __restrict__ cl_col) {
-  int sl = blockIdx.x * FB + threadIdx.x;
-  int ncl = *ncl_ptr;
-  if (sl >= ncl * CL) return;
-  int c = sl / CL, s = sl % CL;
-  // column of cluster c: binary search in col_cl_start
-  int lo = 0, hi = g.ncol[0] * g.ncol[1];
-  while (hi - lo > 1) {
-    int mid = (lo + hi) / 2;
-    if (col_cl_start[mid] <= c) lo = mid; else hi = mid;
-  }
-  int col = lo;
-  if (s == 0) cl_col[c] = col;
-  int cs = col_start[col], ce = col_start[col + 1];
-  int p = cs + (c - col_cl_start[col]) * CL + s;
-  if (p < ce) {
-    int i = sorted_orig[p];
-    double w[3];
-    for (int a = 0; a < 3; ++a) {
-      double f = (double)coords[3 * i + a] * g.invL[a];
-      f -= floor(f);
-      w[a] = f * g.L[a];
-    }
-    xyzq[sl] = {(TC)w[0], (TC)w[1], (TC)w[2], (TC)q[i]};
-    lj[sl] = {(TC)(0.5 * (double)sig[i]), (TC)(2.0 * sqrt((double)eps[i] > 0 ? (double)eps[i] : 0.0))};
-    slot_orig[sl] = i;
-  } else {
-    xyzq[sl] = {(TC)(PAD_STEP * (1.0 + sl)), (TC)PAD_STEP, (TC)PAD_STEP, (TC)0};
-    lj[sl] = {(TC)0, (TC)0};
-    slot_orig[sl] = -1;
-  }
-}
-
-// Reorder the 32 slots of every cluster into four spatially compact groups of 8: sort by x
-// and split in halves, sort each half by y and split, order each group by z (padding slots,
-// keyed +inf, collect at the end; ties broken by slot).  One wave per cluster, lanes 0..31.
-// Also writes slot_of_orig[i] (the final slot of atom i) and the bounding boxes (centre, half
-// extent) of the cluster and of its four groups; an empty group gets half extent -1.  The
-// boxes are widened by a relative epsilon so that float rounding of the tests never drops a pair.
-template <typename TC>
-__global__ __launch_bounds__(FB) void k_reorder_clusters(const int* __restrict__ ncl_ptr, V4<TC>* __restrict__ xyzq,
-                                                         V2<TC>* __restrict__ lj, int* __restrict__ slot_orig,
-                                                         int* __restrict__ slot_of_orig, float4* __restrict__ bb_c,
-                                                         float4* __restrict__ bb_h, float4* __restrict__ sb_c,
-                                                         float4* __restrict__ sb_h) {
+__global__ __launch_bounds__(FB) void k_build_clusters(const TI* __restrict__ coords, const TI* __restrict__ q,
+                                                       const TI* __restrict__ sig, const TI* __restrict__ eps,
+                                                       Geo g, const int* __restrict__ ncl_ptr,
+                                                       const int* __restrict__ col_cl_start,
+                                                       const int* __restrict__ col_start,
+                                                       const int* __restrict__ sorted_orig, V4<TC>* __restrict__ xyzq,
+                                                       V2<TC>* __restrict__ lj, int* __restrict__ slot_orig,
+                                                       int* __restrict__ slot_of_orig, int* __restrict__ cl_col,
+                                                       float4* __restrict__ bb_c, float4* __restrict__ bb_h,
+                                                       float4* __restrict__ sb_c, float4* __restrict__ sb_h) {
   __shared__ float4 s_p[FB / TFF_WAVE][CL];
   const int c = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int w = threadIdx.x / TFF_WAVE;
   if (c >= *ncl_ptr) return;
+  // column of cluster c
+  int lo = 0, hi = g.ncol[0] * g.ncol[1];
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (col_cl_start[mid] <= c) lo = mid; else hi = mid;
+  }
+  const int col = lo;
+  if (lane == 0) cl_col[c] = col;
   const int s = lane & (CL - 1);
-  const int sl = c * CL + s;
-  const V4<TC> p = xyzq[sl];
-  const V2<TC> l = lj[sl];
-  const int o = slot_orig[sl];
+  const int pos = col_start[col] + (c - col_cl_start[col]) * CL + s;
+  const int o = pos < col_start[col + 1] ? sorted_orig[pos] : -1;
+  V4<TC> p;
+  V2<TC> l;
+  if (o >= 0) {
+    double wv[3];
+    wrap_coords(coords, o, g, wv);
+    p = {(TC)wv[0], (TC)wv[1], (TC)wv[2], (TC)q[o]};
+    l = {(TC)(0.5 * (double)sig[o]), (TC)(2.0 * sqrt((double)eps[o] > 0 ? (double)eps[o] : 0.0))};
+  } else {
+    p = {(TC)0, (TC)0, (TC)0, (TC)0};
+    l = {(TC)0, (TC)0};
+  }
   const float inf = 3.0e38f;
   const float kx = o >= 0 ? (float)p.x : inf, ky = o >= 0 ? (float)p.y : inf, kz = o >= 0 ? (float)p.z : inf;
   // rank by x over all 32 slots
@@ -310,9 +302,9 @@ __global__ __launch_bounds__(FB) void k_reorder_clusters(const int* __restrict__
   }
   const int nl = grp * 8 + rz;
   const int ns = c * CL + nl;
-  __builtin_amdgcn_wave_barrier();
   if (lane < CL) {
-    xyzq[ns] = p;
+    // padding slots sit far away, each at its own place (see PAD_STEP)
+    xyzq[ns] = o >= 0 ? p : V4<TC>{(TC)(PAD_STEP * (1.0 + ns)), (TC)PAD_STEP, (TC)PAD_STEP, (TC)0};
     lj[ns] = l;
     slot_orig[ns] = o;
     if (o >= 0) slot_of_orig[o] = ns;
@@ -321,11 +313,11 @@ __global__ __launch_bounds__(FB) void k_reorder_clusters(const int* __restrict__
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // bounding boxes of the reordered slots: lane l holds new slot l & 31
-  const float4 q = s_p[w][lane & (CL - 1)];
-  const bool v = q.w != 0.f;
+  // bounding boxes of the ordered slots: lane l holds slot l & 31
+  const float4 qv = s_p[w][lane & (CL - 1)];
+  const bool v = qv.w != 0.f;
   float mn[3], mx[3], gmn[3], gmx[3];
-  const float pv[3] = {q.x, q.y, q.z};
+  const float pv[3] = {qv.x, qv.y, qv.z};
   for (int a = 0; a < 3; ++a) {
     mn[a] = v ? pv[a] : 3.0e38f;
     mx[a] = v ? pv[a] : -3.0e38f;
@@ -550,9 +542,9 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ nc
 }
 
 // Segments -> one compact cluster-pair list in (ci, segment order) (deterministic), with the
-// sub-tile masks: 4 tiles per wave step, 16 lanes per tile, lane (gi, gj) testing the bounding
-// boxes of group gi of ci and group gj of cj against cut2 (minimum image of the group centres:
-// a lower bound on the distance of every image of every atom pair of the two groups).
+// sub-tile masks: bit gi * 4 + gj is set when the bounding boxes of group gi of ci and group
+// gj of cj are within cut2 (minimum image of the group centres: a lower bound on the distance
+// of every image of every atom pair of the two groups).
 __global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ ncl_ptr, const int2* __restrict__ seg,
                                                       const int* __restrict__ seg_count,
                                                       const int* __restrict__ seg_start, int maxn,
@@ -562,34 +554,36 @@ __global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ nc
   const int ci = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
   if (ci >= *ncl_ptr) return;
   const int n = seg_count[ci], o = seg_start[ci];
-  const int q = lane & 15, gi = q >> 2, gj = q & 3, sub = lane >> 4;
+  // 16 tiles per step: lane = (tile lane >> 2, i-group lane & 3), looping over the j-groups
+  const int gi = lane & 3, sub = lane >> 2;
   const float4 c0 = sb_c[ci * 4 + gi], h0 = sb_h[ci * 4 + gi];
   const float L0 = (float)g.L[0], L1 = (float)g.L[1], L2 = (float)g.L[2];
   const float i0 = (float)g.invL[0], i1 = (float)g.invL[1], i2 = (float)g.invL[2];
-  for (int k0 = 0; k0 < n; k0 += 4) {
+  for (int k0 = 0; k0 < n; k0 += 16) {
     const int k = k0 + sub;
-    bool ok = false;
+    unsigned bits = 0;
     int2 e = make_int2(0, 0);
     if (k < n) {
       e = seg[(long long)ci * maxn + k];
       const int cj = e.x;
-      const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
-      if (h0.x >= 0.f && h1.x >= 0.f && (ci != cj || gj >= gi)) {
-        float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
-        dx -= L0 * rintf(dx * i0);
-        dy -= L1 * rintf(dy * i1);
-        dz -= L2 * rintf(dz * i2);
-        const float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
-        const float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
-        const float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
-        ok = gx * gx + gy * gy + gz * gz <= cut2;
+#pragma unroll
+      for (int gj = 0; gj < 4; ++gj) {
+        const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
+        if (h0.x >= 0.f && h1.x >= 0.f && (ci != cj || gj >= gi)) {
+          float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
+          dx -= L0 * rintf(dx * i0);
+          dy -= L1 * rintf(dy * i1);
+          dz -= L2 * rintf(dz * i2);
+          const float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
+          const float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
+          const float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
+          if (gx * gx + gy * gy + gz * gz <= cut2) bits |= 1u << (gi * 4 + gj);
+        }
       }
     }
-    const unsigned long long bm = __ballot(ok);
-    if (k < n && q == 0) {
-      const unsigned m = (unsigned)((bm >> (lane & 48)) & 0xffffull);
-      pairs[o + k] = make_int4(ci, e.x, (int)((unsigned)e.y | m), 0);
-    }
+    bits |= __shfl_xor(bits, 1, TFF_WAVE);
+    bits |= __shfl_xor(bits, 2, TFF_WAVE);
+    if (k < n && gi == 0) pairs[o + k] = make_int4(ci, e.x, (int)((unsigned)e.y | bits), 0);
   }
 }
 
@@ -1349,12 +1343,10 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
                      w.sorted_orig);
   hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(256), 0, st, coords, g, w.col_start, w.sorted_orig);
   // 2. cluster arrays
-  hipLaunchKernelGGL((k_fill_clusters<TI, TC>), dim3(nbS), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
-                     w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
-                     w.cl_col);
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
-  hipLaunchKernelGGL(k_reorder_clusters<TC>, dim3(nbW), dim3(FB), 0, st, w.ncl, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj,
-                     w.slot_orig, w.slot_of_orig, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
+  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbW), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
+                     w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
+                     w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
   hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
                      col, w.excl_part, status);
   // 3. cluster pairs
