@@ -1221,6 +1221,9 @@ int tile_waves(const Geo& g) {
   return (int)w;
 }
 
+// z extent of the reciprocal-space array: even K2 uses the real-input half spectrum
+int spectrum_nz(const int K[3]) { return K[2] % 2 == 0 ? K[2] / 2 + 1 : K[2]; }
+
 struct FusedWs {
   int *col_of, *off_in_col, *col_count, *col_start, *sorted_orig, *col_cl_start, *ncl, *cl_col, *slot_orig,
       *excl_part, *ncp, *seg_count, *seg_start, *slot_of_orig;
@@ -1276,7 +1279,7 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   w.eacc = (unsigned long long*)take(8 * 2 * E_SLOTS);
   w.grid = (unsigned long long*)take(8 * ng);
   w.work = take(2 * tc * ng);
-  PassCfg px = pass_cfg(1, K[0], K[1] * K[2], 2 * tc);
+  PassCfg px = pass_cfg(1, K[0], K[1] * spectrum_nz(K), 2 * tc);
   w.nk = px.tiles;
   w.kparts = (double*)take(8 * (size_t)w.nk);
   w.nq = (int)((nn + FB - 1) / FB);
@@ -1401,22 +1404,38 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   cv.grid_fixed = 1;
   cv.grid_inv_scale = 1.0 / grid_scale<TC>();
   const size_t ce = 2 * sizeof(TC);
+  // real-input transforms along z when K2 is even (half spectrum, K2 / 2 + 1 planes), else complex
+  const bool rz = K[2] % 2 == 0;
+  const int nzc = spectrum_nz(K);
+  cv.nz = nzc;
+  cv.half = rz ? 1 : 0;
   PassCfg cz = pass_cfg(K[0] * K[1], K[2], 1, ce);
-  PassCfg cy = pass_cfg(K[0], K[1], K[2], ce);
-  PassCfg cxp = pass_cfg(1, K[0], K[1] * K[2], ce);
+  PassCfg cy = pass_cfg(K[0], K[1], nzc, ce);
+  PassCfg cxp = pass_cfg(1, K[0], K[1] * nzc, ce);
+  FftPlan pzh;
+  make_plan(rz ? K[2] / 2 : 1, pzh);
+  const ZPlanCfg zc = zcfg(K[0] * K[1], K[2], ce);
   TC* phi = (TC*)w.grid;  // the real potential grid reuses the charge grid's memory after the z pass
   {
     void* tok = tff_prof_begin("fused_fft", st);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_FWD>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.grid, w.work,
-                       cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
+    if (rz)
+      hipLaunchKernelGGL(k_fft_z_r2c<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, st, w.grid, (cx<TC>*)w.work,
+                         K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh, 1, cv.grid_inv_scale);
+    else
+      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_FWD>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.grid, w.work,
+                         cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
     hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_FWD>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, st, w.work, w.work,
                        cy.pa, pl[1], cv, mx, my, mz, 1, w.kparts);
     hipLaunchKernelGGL((k_fft_pass<TC, PASS_X_CONV>), dim3(cxp.tiles), dim3(FFT_THREADS), cxp.lds, st, w.work,
                        w.work, cxp.pa, pl[0], cv, mx, my, mz, 1, w.kparts);
     hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_INV>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, st, w.work, w.work,
                        cy.pa, pl[1], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_INV>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.work, phi,
-                       cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
+    if (rz)
+      hipLaunchKernelGGL(k_fft_z_c2r<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, st, (const cx<TC>*)w.work, phi,
+                         K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh);
+    else
+      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_INV>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.work, phi,
+                         cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
     tff_prof_end(tok, st);
   }
   {

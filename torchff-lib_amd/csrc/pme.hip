@@ -257,6 +257,8 @@ int pme_energy(const T* coords, const T* q, long long n, const TffBox& box, cons
   cv.alpha = alpha;
   cv.grid_fixed = 0;
   cv.grid_inv_scale = 1.0;
+  cv.nz = K[2];
+  cv.half = 0;
   PassCfg cz = pass_cfg(K[0] * K[1], K[2], 1, ce);
   PassCfg cy = pass_cfg(K[0], K[1], K[2], ce);
   PassCfg cxp = pass_cfg(1, K[0], K[1] * K[2], ce);

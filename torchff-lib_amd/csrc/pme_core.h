@@ -213,7 +213,35 @@ struct ConvArgs {
   double alpha;
   int grid_fixed;         // forward z pass input: 0 = float64 grid, 1 = int64 fixed-point grid
   double grid_inv_scale;  // fixed-point scale of that grid (value = integer * grid_inv_scale)
+  int nz;                 // z extent of the spectrum: K[2] (full) or K[2] / 2 + 1 (real-input half)
+  int half;               // 1: half spectrum along z (energy weight 2 on kz in (0, K[2] / 2))
 };
+
+// cos(pi x), sin(pi x) in the transform's precision
+template <typename T>
+__device__ __forceinline__ void twiddle_pi(double x, T& c, T& s) {
+  if constexpr (sizeof(T) == 4) {
+    float sn, cs;
+    sincospif((float)x, &sn, &cs);
+    c = cs;
+    s = sn;
+  } else {
+    double sn, cs;
+    sincospi(x, &sn, &cs);
+    c = cs;
+    s = sn;
+  }
+}
+
+// tw[m] = exp(-2 pi i m / K), m < K (LDS; the caller synchronises before use)
+template <typename T>
+__device__ __forceinline__ void fill_twiddles(cx<T>* tw, int K) {
+  for (int m = threadIdx.x; m < K; m += blockDim.x) {
+    T c, s;
+    twiddle_pi(-2.0 * (double)m / (double)K, c, s);
+    tw[m] = {c, s};
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ T signed_m(int m, int K) {
@@ -233,17 +261,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
   cx<T>* bufA = tw + K;
   cx<T>* bufB = bufA + nl * LS;
   // twiddles exp(-2 pi i m / K)
-  for (int m = threadIdx.x; m < K; m += blockDim.x) {
-    if constexpr (sizeof(T) == 4) {
-      float sn, cs;
-      sincospif(-2.0f * (float)m / (float)K, &sn, &cs);
-      tw[m] = {cs, sn};
-    } else {
-      double sn, cs;
-      sincospi(-2.0 * (double)m / (double)K, &sn, &cs);
-      tw[m] = {(T)cs, (T)sn};
-    }
-  }
+  fill_twiddles(tw, K);
   const int ntc = (pa.C + pa.CB - 1) / pa.CB;
   const int tile = blockIdx.x;
   const int a0 = (tile / ntc) * pa.AB;
@@ -287,12 +305,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
     // lines are (y, z) = c; along the line k = mx
     double e_acc = 0.0;
     const T two_pi = (T)6.283185307179586;
+    const float inv_nz = 1.0f / (float)cv.nz;
     for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
       int line = fdiv(idx, inv_k);
       int k = idx - line * K;
       int c = c0 + line;  // AB == 1 for this pass
       if (c >= pa.C) continue;
-      int my = c / cv.K[2], mz = c - my * cv.K[2];
+      int my = fdiv(c, inv_nz), mz = c - my * cv.nz;
       T fx = signed_m<T>(k, cv.K[0]), fy = signed_m<T>(my, cv.K[1]), fz = signed_m<T>(mz, cv.K[2]);
       T kx = two_pi * (fx * (T)cv.inv[0] + fy * (T)cv.inv[1] + fz * (T)cv.inv[2]);
       T ky = two_pi * (fx * (T)cv.inv[3] + fy * (T)cv.inv[4] + fz * (T)cv.inv[5]);
@@ -305,7 +324,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
         T th = modx[k] * mody[my] * modz[zi];
         G = (T)(4.0 * 3.14159265358979323846 / cv.vol) / ksq * exp(-ksq / (T)(4.0 * cv.alpha * cv.alpha)) / (th * th);
       }
-      e_acc += 0.5 * (double)G * ((double)s.x * (double)s.x + (double)s.y * (double)s.y);
+      // half spectrum: kz in (0, K/2) stands for kz and -kz
+      const double wz = cv.half && mz != 0 && 2 * mz != cv.K[2] ? 1.0 : 0.5;
+      e_acc += wz * (double)G * ((double)s.x * (double)s.x + (double)s.y * (double)s.y);
       s.x *= G;
       s.y *= G;
     }
@@ -329,6 +350,114 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
         reinterpret_cast<cx<T>*>(out_v)[g] = v;
     }
   }
+}
+
+// Real-to-complex transform along the contiguous (z) axis of a real grid of nlines lines of
+// K2 reals, K2 even: a line is taken as M = K2 / 2 complex values z_n = x_2n + i x_2n+1,
+// transformed by an M-point FFT (Z), and untangled into X_k = E_k + w^k O_k for k = 0..M with
+// E_k = (Z_k + conj Z_M-k) / 2, O_k = (Z_k - conj Z_M-k) / 2i, w = exp(-2 pi i / K2) (Z_M = Z_0).
+// Output: nlines x (M + 1) complex.  Input: int64 fixed point (grid_fixed) or float64.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_z_r2c(const void* __restrict__ in_v, cx<T>* __restrict__ out,
+                                                           int nlines, int K2, int AB, int ab_shift, FftPlan plan,
+                                                           int grid_fixed, double grid_inv_scale) {
+  extern __shared__ unsigned char lds_raw[];
+  const int M = K2 >> 1, LS = M + 1;
+  cx<T>* tw = reinterpret_cast<cx<T>*>(lds_raw);
+  cx<T>* bufA = tw + M;
+  cx<T>* bufB = bufA + AB * LS;
+  fill_twiddles(tw, M);
+  const int l0 = blockIdx.x * AB;
+  const float inv_m = 1.0f / (float)M, inv_m1 = 1.0f / (float)(M + 1);
+  for (int idx = threadIdx.x; idx < AB * M; idx += blockDim.x) {
+    const int ab = fdiv(idx, inv_m), n = idx - ab * M, line = l0 + ab;
+    cx<T> v = {(T)0, (T)0};
+    if (line < nlines) {
+      const long long g = (long long)line * K2 + 2 * n;
+      if (grid_fixed) {
+        const long long* gi = reinterpret_cast<const long long*>(in_v);
+        v = {(T)((double)gi[g] * grid_inv_scale), (T)((double)gi[g + 1] * grid_inv_scale)};
+      } else {
+        const double* gd = reinterpret_cast<const double*>(in_v);
+        v = {(T)gd[g], (T)gd[g + 1]};
+      }
+    }
+    bufA[ab * LS + n] = v;
+  }
+  __syncthreads();
+  const cx<T>* res = fft_lines<T, -1>(bufA, bufB, tw, M, LS, ab_shift, plan);
+  for (int idx = threadIdx.x; idx < AB * (M + 1); idx += blockDim.x) {
+    const int ab = fdiv(idx, inv_m1), k = idx - ab * (M + 1), line = l0 + ab;
+    if (line >= nlines) continue;
+    const cx<T> zk = res[ab * LS + (k == M ? 0 : k)];
+    const cx<T> zm = res[ab * LS + (k == 0 ? 0 : M - k)];
+    const T ex = (T)0.5 * (zk.x + zm.x), ey = (T)0.5 * (zk.y - zm.y);
+    const T ox = (T)0.5 * (zk.y + zm.y), oy = (T)-0.5 * (zk.x - zm.x);
+    T c, sn;
+    twiddle_pi(-2.0 * (double)k / (double)K2, c, sn);
+    out[(long long)line * (M + 1) + k] = {ex + c * ox - sn * oy, ey + c * oy + sn * ox};
+  }
+}
+
+// Inverse of k_fft_z_r2c without normalisation: X_k (k = 0..M) of real lines -> x_n =
+// sum over the full spectrum of X_k exp(2 pi i k n / K2), via Z_k = (X_k + conj X_M-k)
+// + i exp(2 pi i k / K2) (X_k - conj X_M-k), z = M-point inverse FFT of Z, x_2n + i x_2n+1 = z_n.
+template <typename T>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_z_c2r(const cx<T>* __restrict__ in, T* __restrict__ out,
+                                                           int nlines, int K2, int AB, int ab_shift, FftPlan plan) {
+  extern __shared__ unsigned char lds_raw[];
+  const int M = K2 >> 1, LS = M + 1;
+  cx<T>* tw = reinterpret_cast<cx<T>*>(lds_raw);
+  cx<T>* bufA = tw + M;
+  cx<T>* bufB = bufA + AB * LS;
+  fill_twiddles(tw, M);
+  const int l0 = blockIdx.x * AB;
+  const float inv_m = 1.0f / (float)M, inv_m1 = 1.0f / (float)(M + 1);
+  for (int idx = threadIdx.x; idx < AB * (M + 1); idx += blockDim.x) {
+    const int ab = fdiv(idx, inv_m1), k = idx - ab * (M + 1), line = l0 + ab;
+    bufA[ab * LS + k] = line < nlines ? in[(long long)line * (M + 1) + k] : cx<T>{(T)0, (T)0};
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < AB * M; idx += blockDim.x) {
+    const int ab = fdiv(idx, inv_m), k = idx - ab * M;
+    const cx<T> xk = bufA[ab * LS + k], xm = bufA[ab * LS + M - k];
+    const T ax = xk.x + xm.x, ay = xk.y - xm.y;
+    const T bx = xk.x - xm.x, by = xk.y + xm.y;
+    T c, sn;
+    twiddle_pi(2.0 * (double)k / (double)K2, c, sn);
+    const T wx = c * bx - sn * by, wy = c * by + sn * bx;
+    bufB[ab * LS + k] = {ax - wy, ay + wx};
+  }
+  __syncthreads();
+  const cx<T>* res = fft_lines<T, 1>(bufB, bufA, tw, M, LS, ab_shift, plan);
+  for (int idx = threadIdx.x; idx < AB * M; idx += blockDim.x) {
+    const int ab = fdiv(idx, inv_m), n = idx - ab * M, line = l0 + ab;
+    if (line >= nlines) continue;
+    const cx<T> z = res[ab * LS + n];
+    out[(long long)line * K2 + 2 * n] = z.x;
+    out[(long long)line * K2 + 2 * n + 1] = z.y;
+  }
+}
+
+struct ZPlanCfg {
+  int AB, ab_shift, blocks;
+  size_t lds;
+};
+// tiles of the real z passes: AB lines (a power of two, at most 16) of M + 1 complex, two buffers
+ZPlanCfg zcfg(int nlines, int K2, size_t celem) {
+  ZPlanCfg c{};
+  const int M = K2 / 2;
+  const size_t per_line = 2 * (size_t)(M + 1) * celem;
+  int lines = (int)((FFT_LDS_BUDGET - (size_t)M * celem) / per_line);
+  if (lines > 16) lines = 16;
+  if (lines < 1) lines = 1;
+  c.AB = 1;
+  while (c.AB * 2 <= lines) c.AB *= 2;
+  c.ab_shift = 0;
+  while ((1 << c.ab_shift) < c.AB) ++c.ab_shift;
+  c.blocks = (nlines + c.AB - 1) / c.AB;
+  c.lds = (size_t)M * celem + 2 * (size_t)c.AB * (M + 1) * celem;
+  return c;
 }
 
 struct PassCfg {
