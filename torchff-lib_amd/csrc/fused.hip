@@ -23,6 +23,8 @@
 // the order of summation); the reciprocal-space energy is a float64 k-space sum.
 // TC = float is the mixed-precision mode, TC = double the all-float64 mode.
 
+#include <mutex>
+
 #include "pme_core.h"
 #include "tff_common.h"
 #include "torchff_hip.h"
@@ -1139,18 +1141,21 @@ __global__ __launch_bounds__(FB) void k_gather_slots(const int* __restrict__ ncl
   const double gxr = d0 * g.inv[0] + d1 * g.inv[1] + d2 * g.inv[2];
   const double gyr = d0 * g.inv[3] + d1 * g.inv[4] + d2 * g.inv[5];
   const double gzr = d0 * g.inv[6] + d1 * g.inv[7] + d2 * g.inv[8];
-  // force = -dE/dr, added to this slot's accumulator (one thread per slot: no atomics needed)
-  facc[3 * sl + 0] += (unsigned long long)to_fx(-kq * gxr, F_SCALE);
-  facc[3 * sl + 1] += (unsigned long long)to_fx(-kq * gyr, F_SCALE);
-  facc[3 * sl + 2] += (unsigned long long)to_fx(-kq * gzr, F_SCALE);
+  // force = -dE/dr into the slot's reciprocal-space accumulator (one thread per slot; the pair
+  // kernel, which may run concurrently, adds into the other accumulator)
+  facc[3 * sl + 0] = (unsigned long long)to_fx(-kq * gxr, F_SCALE);
+  facc[3 * sl + 1] = (unsigned long long)to_fx(-kq * gyr, F_SCALE);
+  facc[3 * sl + 2] = (unsigned long long)to_fx(-kq * gzr, F_SCALE);
 }
 
 // ---------------------------------------------------------------- 6. finalize
-// Forces back in the caller's atom order; block 0 also sums the energy terms:
+// Forces (pair + reciprocal accumulators) back in the caller's atom order; block 0 also sums
+// the energy terms:
 // out = [Lennard-Jones, real-space Coulomb, reciprocal, self, total].
 template <typename TI>
 __global__ __launch_bounds__(FB) void k_finalize(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
-                                                 const unsigned long long* __restrict__ facc, TI* __restrict__ forces,
+                                                 const unsigned long long* __restrict__ facc,
+                                                 const unsigned long long* __restrict__ facc_pme, TI* __restrict__ forces,
                                                  const unsigned long long* __restrict__ eacc,
                                                  const double* __restrict__ kparts, int nk,
                                                  const double* __restrict__ qparts, int nq, double coul,
@@ -1159,7 +1164,8 @@ __global__ __launch_bounds__(FB) void k_finalize(const int* __restrict__ ncl_ptr
   if (sl < *ncl_ptr * CL) {
     const int i = slot_orig[sl];
     if (i >= 0)
-      for (int a = 0; a < 3; ++a) forces[3 * i + a] = (TI)((double)(long long)facc[3 * sl + a] / F_SCALE);
+      for (int a = 0; a < 3; ++a)
+        forces[3 * i + a] = (TI)((double)(long long)(facc[3 * sl + a] + facc_pme[3 * sl + a]) / F_SCALE);
   }
   if (blockIdx.x != 0) return;
   const double ek = tff_sum_partials(kparts, nk);
@@ -1231,7 +1237,7 @@ struct FusedWs {
   void *xyzq, *lj;
   float4 *bb_c, *bb_h, *sb_c, *sb_h;
   int4* cpairs;
-  unsigned long long *facc, *grid;
+  unsigned long long *facc, *facc_pme, *grid;
   void* work;
   double *kparts, *qparts;
   unsigned long long* eacc;
@@ -1275,6 +1281,7 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   w.seg_count = (int*)take(4 * (size_t)g.ncl_max);
   w.seg_start = (int*)take(4 * (size_t)g.ncl_max);
   w.facc = (unsigned long long*)take(8 * 3 * slots);
+  w.facc_pme = (unsigned long long*)take(8 * 3 * slots);
   w.nw_tiles = tile_waves(g);
   w.eacc = (unsigned long long*)take(8 * 2 * E_SLOTS);
   w.grid = (unsigned long long*)take(8 * ng);
@@ -1299,7 +1306,7 @@ int launch_pme_slots(int order, const FusedWs& w, const Geo& g, const PmeGeom& p
                        w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status);                      \
   else                                                                                                          \
     hipLaunchKernelGGL((k_gather_slots<TC, P>), dim3(nb), dim3(FB), 0, st, w.ncl, phi, (const V4<TC>*)w.xyzq,   \
-                       w.slot_orig, pg, coul, w.facc)
+                       w.slot_orig, pg, coul, w.facc_pme)
   switch (order) {
     case 3: TFF_SP(3); break;
     case 4: TFF_SP(4); break;
@@ -1313,11 +1320,50 @@ int launch_pme_slots(int order, const FusedWs& w, const Geo& g, const PmeGeom& p
   return TFF_OK;
 }
 
+// Per-device auxiliary stream (default priority) and fork / join events.  The fork makes the
+// auxiliary stream wait for the work enqueued so far on the caller's stream; the join makes
+// the caller's stream wait for the auxiliary work, so the call stays ordered on the caller's
+// stream (and can be captured into a graph).  The mutex covers each record / wait pair.
+struct AuxStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_aux_mu;
+AuxStream g_aux[64];
+
+hipStream_t aux_stream_fork(hipStream_t st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  AuxStream& a = g_aux[dev];
+  if (!a.s) {
+    if (hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) return nullptr;
+  }
+  if (hipEventRecord(a.fork, st) != hipSuccess) return nullptr;
+  if (hipStreamWaitEvent(a.s, a.fork, 0) != hipSuccess) return nullptr;
+  return a.s;
+}
+
+void aux_stream_join(hipStream_t st) {
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  AuxStream& a = g_aux[dev];
+  hipEventRecord(a.join, a.s);
+  hipStreamWaitEvent(st, a.join, 0);
+}
+
 template <typename TI, typename TC>
 int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long long n, const TffBox& box,
               const Geo& g, const int64_t* row_ptr, const int64_t* col, double cutoff, double list_cutoff,
               double alpha, double coul, const int K[3], int order, const TC* mx, const TC* my, const TC* mz,
               double* energies, TI* forces, int* status, const FusedWs& w, hipStream_t st) {
+  if (order < 3 || order > 8) {  // checked before any launch: the stream fork below must be joined
+    tff_set_error("fused: unsupported spline order %d (3..8)", order);
+    return TFF_ERR_UNSUPPORTED;
+  }
   const unsigned nbA = (unsigned)((n + FB - 1) / FB);
   const int nslots = g.ncl_max * CL;
   const unsigned nbS = (unsigned)((nslots + FB - 1) / FB);
@@ -1352,48 +1398,19 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
                      w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
   hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
                      col, w.excl_part, status);
-  // 3. cluster pairs
-  {
-    void* tok = tff_prof_begin("fused_cluster_pairs", st);
-    float rl2 = (float)(list_cutoff * list_cutoff);
-    hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c,
-                       w.bb_h, g, rl2, w.excl_part, w.seg, w.seg_count, status);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
-    // sub-tiles are pruned at the interaction cutoff itself: pairs between rc and the list
-    // cutoff contribute nothing to this evaluation
-    hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
-                       w.sb_c, w.sb_h, g, (float)(cutoff * cutoff), w.cpairs);
-    tff_prof_end(tok, st);
+  hipStream_t sp = aux_stream_fork(st);
+  if (!sp) {
+    tff_set_error("fused: could not create the auxiliary stream");
+    return TFF_ERR_LAUNCH;
   }
-  // 4. tiles
-  {
-    TileArgs<TC> ta;
-    for (int a = 0; a < 3; ++a) {
-      ta.L[a] = (TC)g.L[a];
-      ta.invL[a] = (TC)g.invL[a];
-    }
-    ta.rc2 = (TC)(cutoff * cutoff);
-    ta.alpha = (TC)alpha;
-    ta.coul = (TC)coul;
-    void* tok = tff_prof_begin("fused_tiles", st);
-    if constexpr (sizeof(TC) == 4)
-      hipLaunchKernelGGL(k_tiles_f32, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
-                         g.pair_cap, (const V4<float>*)w.xyzq, (const V2<float>*)w.lj, w.excl_part, ta, w.facc,
-                         w.eacc);
-    else
-      hipLaunchKernelGGL(k_tiles_f64, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
-                         g.pair_cap, (const V4<double>*)w.xyzq, (const V2<double>*)w.lj, w.excl_part, ta, w.facc,
-                         w.eacc);
-    tff_prof_end(tok, st);
-  }
-  // 5. PME
+  // 3. PME on the auxiliary stream (needs only the cluster arrays), concurrent with 4-5
   PmeGeom pg = make_geom(box, K, order, alpha);
   FftPlan pl[3];
   for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
   {
-    void* tok = tff_prof_begin("fused_spread", st);
-    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, nullptr, coul, true, status, st);
-    tff_prof_end(tok, st);
+    void* tok = tff_prof_begin("fused_spread", sp);
+    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, nullptr, coul, true, status, sp);
+    tff_prof_end(tok, sp);
     if (rc) return rc;
   }
   ConvArgs cv;
@@ -1417,35 +1434,70 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   const ZPlanCfg zc = zcfg(K[0] * K[1], K[2], ce);
   TC* phi = (TC*)w.grid;  // the real potential grid reuses the charge grid's memory after the z pass
   {
-    void* tok = tff_prof_begin("fused_fft", st);
+    void* tok = tff_prof_begin("fused_fft", sp);
     if (rz)
-      hipLaunchKernelGGL(k_fft_z_r2c<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, st, w.grid, (cx<TC>*)w.work,
+      hipLaunchKernelGGL(k_fft_z_r2c<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, sp, w.grid, (cx<TC>*)w.work,
                          K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh, 1, cv.grid_inv_scale);
     else
-      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_FWD>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.grid, w.work,
+      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_FWD>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, sp, w.grid, w.work,
                          cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_FWD>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, st, w.work, w.work,
+    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_FWD>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, sp, w.work, w.work,
                        cy.pa, pl[1], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_X_CONV>), dim3(cxp.tiles), dim3(FFT_THREADS), cxp.lds, st, w.work,
+    hipLaunchKernelGGL((k_fft_pass<TC, PASS_X_CONV>), dim3(cxp.tiles), dim3(FFT_THREADS), cxp.lds, sp, w.work,
                        w.work, cxp.pa, pl[0], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_INV>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, st, w.work, w.work,
+    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_INV>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, sp, w.work, w.work,
                        cy.pa, pl[1], cv, mx, my, mz, 1, w.kparts);
     if (rz)
-      hipLaunchKernelGGL(k_fft_z_c2r<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, st, (const cx<TC>*)w.work, phi,
+      hipLaunchKernelGGL(k_fft_z_c2r<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, sp, (const cx<TC>*)w.work, phi,
                          K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh);
     else
-      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_INV>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.work, phi,
+      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_INV>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, sp, w.work, phi,
                          cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
-    tff_prof_end(tok, st);
+    tff_prof_end(tok, sp);
   }
   {
-    void* tok = tff_prof_begin("fused_gather", st);
-    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, phi, coul, false, status, st);
-    tff_prof_end(tok, st);
+    void* tok = tff_prof_begin("fused_gather", sp);
+    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, phi, coul, false, status, sp);
+    tff_prof_end(tok, sp);
     if (rc) return rc;
   }
-  // 6. outputs
-  hipLaunchKernelGGL(k_finalize<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, forces, w.eacc,
+  // 4. cluster pairs
+  {
+    void* tok = tff_prof_begin("fused_cluster_pairs", st);
+    float rl2 = (float)(list_cutoff * list_cutoff);
+    hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c,
+                       w.bb_h, g, rl2, w.excl_part, w.seg, w.seg_count, status);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
+    // sub-tiles are pruned at the interaction cutoff itself: pairs between rc and the list
+    // cutoff contribute nothing to this evaluation
+    hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
+                       w.sb_c, w.sb_h, g, (float)(cutoff * cutoff), w.cpairs);
+    tff_prof_end(tok, st);
+  }
+  // 5. tiles
+  {
+    TileArgs<TC> ta;
+    for (int a = 0; a < 3; ++a) {
+      ta.L[a] = (TC)g.L[a];
+      ta.invL[a] = (TC)g.invL[a];
+    }
+    ta.rc2 = (TC)(cutoff * cutoff);
+    ta.alpha = (TC)alpha;
+    ta.coul = (TC)coul;
+    void* tok = tff_prof_begin("fused_tiles", st);
+    if constexpr (sizeof(TC) == 4)
+      hipLaunchKernelGGL(k_tiles_f32, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
+                         g.pair_cap, (const V4<float>*)w.xyzq, (const V2<float>*)w.lj, w.excl_part, ta, w.facc,
+                         w.eacc);
+    else
+      hipLaunchKernelGGL(k_tiles_f64, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
+                         g.pair_cap, (const V4<double>*)w.xyzq, (const V2<double>*)w.lj, w.excl_part, ta, w.facc,
+                         w.eacc);
+    tff_prof_end(tok, st);
+  }
+  // 6. outputs, after the reciprocal-space work
+  aux_stream_join(st);
+  hipLaunchKernelGGL(k_finalize<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, w.facc_pme, forces, w.eacc,
                      w.kparts, w.nk, w.qparts, w.nq, coul, alpha, energies);
   TFF_CHECK_LAUNCH("fused energy+forces");
   return TFF_OK;
