@@ -51,7 +51,59 @@ def parse():
     ap.add_argument("--dropin-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", default="", help="comma list of paths to run (mixed,fp64,dropin)")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes (to calibrate HBM byte counters)")
     return ap.parse_args()
+
+
+def calibrate_copy(mb, steps):
+    """A plain device-to-device copy of known size: the FETCH_SIZE / WRITE_SIZE counters read
+    for it give the factor between counter KiB and bytes moved."""
+    n = int(mb * 1e6) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    print(json.dumps({"calibration": "device copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "ms": sec * 1e3, "GB/s": 8 * n / sec / 1e9}))
+
+
+def tile_traffic():
+    """HBM bytes per launch of the pair kernel from the committed counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "fused_tiles_traffic_water96k.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            line = line.strip().lstrip("- ")
+            if line.startswith(("fetch_bytes_per_launch:", "write_bytes_per_launch:")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = float(v.strip().split()[0])
+    if len(vals) != 2:
+        return None
+    return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
+
+
+def pme_stage_bytes(w):
+    """Algorithmic HBM bytes per evaluation of the reciprocal-space stages (N atoms, K^3 grid,
+    float32 grid, int64 fixed-point charge grid, real-input half spectrum along z)."""
+    n = w["natoms"]
+    k = w["K"]
+    half = k * k * (k // 2 + 1) * 8          # complex float half spectrum
+    grid64, grid32 = 8 * k ** 3, 4 * k ** 3
+    return {
+        "spread": n * 16 + grid64,                                   # read x,y,z,q; write the int64 grid
+        "fft": (grid64 + half) + 4 * half + 4 * half + (half + grid32),  # z r2c, y, x+conv+x, y, z c2r
+        "gather": grid32 + n * 16 + n * 24,                          # read potential + atoms; write forces
+    }
 
 
 class DropIn:
@@ -147,6 +199,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy > 0:
+        calibrate_copy(args.calibrate_copy, args.steps)
+        return
     from torchff.workloads import make_workload
     dev = "cuda:0"
     only = set(args.only.split(",")) if args.only else {"mixed", "fp64", "dropin"}
@@ -181,7 +236,7 @@ def main():
         t = kern["fused_tiles"]["avg_ms"]
         achieved = p_in * PAIR_FLOPS / (t * 1e-3) / 1e12
         roof = {"kernel": "fused_tiles", "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "avg_ms": t,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": tile_traffic(), "avg_ms": t,
                 "units": f"{p_in} in-cutoff pairs x {PAIR_FLOPS} FLOP per launch",
                 "tiles": tiles, "pair_slot_efficiency": p_in / (tiles * 1024.0)}
     out = {
@@ -213,6 +268,14 @@ def main():
     if head == "mixed":
         out["kernels_ms"] = {k: v["avg_ms"] for k, v in kern.items()}
         out["roofline"] = roof
+        sb = pme_stage_bytes(w)
+        out["pme_stages"] = {
+            "note": "reciprocal space runs on a second stream concurrently with the pair search and pair "
+                    "kernel: stage times are wall times under that sharing (bandwidths are lower bounds)",
+            **{st: {"bytes": b, "ms": kern[f"fused_{st}"]["avg_ms"],
+                    "GB/s": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9,
+                    "frac_of_hbm_peak": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+               for st, b in sb.items()}}
     if not args.no_cpu_baseline:
         from oracle import composite
         threads = os.cpu_count() or 1
