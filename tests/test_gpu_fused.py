@@ -85,3 +85,20 @@ def test_fused_small_odd_box():
     scale = sum(abs(v) for v in terms.values())
     assert abs(Ed - E) < 1e-9 * scale
     assert _rms_rel(Fd, F) < 1e-9
+
+
+@pytest.mark.parametrize("precision,te,tf", [("fp64", 1e-9, 1e-9), ("mixed", 1e-7, 1e-4)])
+def test_fused_tiny_grid(precision, te, tf):
+    # 64 waters at 1 g/cm3 (L ~ 1.24 nm), rc 0.6, a 10^3 grid: every axis is shorter than a
+    # spreading brick plus a stencil, so the stencil wraps into both ends of a brick
+    from torchff.workloads import box_length_for_density
+    n = 64
+    L = box_length_for_density(n)
+    w = water_box(n, L, seed=33)
+    w.update(cutoff=0.6, skin=0.0, K=10, order=5, alpha=ewald_alpha(0.6))
+    w["name"] = "water192"
+    E, F, terms, _ = composite.energy_forces(w)
+    Ed, Fd, _ = _run(w, precision)
+    scale = sum(abs(v) for v in terms.values())
+    assert abs(Ed - E) < te * scale, (Ed, E)
+    assert _rms_rel(Fd, F) < tf
