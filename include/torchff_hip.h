@@ -145,6 +145,30 @@ int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords,
                             void* forces_out, int32_t* status_out, void* workspace, size_t workspace_bytes,
                             tff_stream_t stream);
 
+/* Vector-Jacobian product of tff_fused_energy_forces, for force matching: given upstream
+ * gradients grad_energy (device, 1 double: dL/dE; NULL = 0) and grad_forces (device, io_dtype
+ * N x 3: dL/dF; NULL = 0), returns dL/dcoords, dL/dcharges, dL/dsigma, dL/depsilon (device,
+ * io_dtype; each may be NULL), i.e. gE dE/dtheta - d(sum_i v_i . dE/dr_i)/dtheta with
+ * v = grad_forces: the second derivatives of the energy that the reference obtains only from
+ * its pure-PyTorch path by double backward (torchff/multipoles.py:244-341, torchff/vdw.py:86-111,
+ * torchff/pme_ref.py:11-469 with create_graph; its CUDA ops' backward only scales stored first
+ * derivatives, e.g. csrc/nonbonded/nonbonded_atom_pairs_cuda.cu:143-260).  The cluster pair list
+ * is rebuilt, not stored from the forward call.  dL/depsilon_i is 0 for pairs with
+ * eps_i eps_j = 0 (as the forward's first derivatives).  Same arguments and constraints as
+ * tff_fused_energy_forces otherwise; workspace sized by tff_fused_backward_workspace_bytes. */
+size_t tff_fused_backward_workspace_bytes(int compute_dtype, long long natoms, const double* box_host,
+                                          double list_cutoff, int K0, int K1, int K2);
+int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void* coords, const void* charges,
+                                     const void* sigma, const void* epsilon, long long natoms,
+                                     const double* box_host, const int64_t* excl_row_ptr,
+                                     const int64_t* excl_col, double cutoff, double list_cutoff,
+                                     double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
+                                     const void* xmoduli, const void* ymoduli, const void* zmoduli,
+                                     const double* grad_energy, const void* grad_forces, void* grad_coords,
+                                     void* grad_charges, void* grad_sigma, void* grad_epsilon,
+                                     int32_t* status_out, void* workspace, size_t workspace_bytes,
+                                     tff_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

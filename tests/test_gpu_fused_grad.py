@@ -1,0 +1,124 @@
+"""Force-matching backward of the fused path (GPU) against the float64 oracle and finite differences.
+
+The oracle (``oracle.composite.force_matching``, pinned by finite differences in
+tests/test_oracle_forcematch.py) gives L = mean((F - T)^2) and dL/d(coords, q, sigma, eps) by
+float64 double backward.  Tolerances (relative RMS over each gradient array): fp64 1e-8, mixed
+(float32 arithmetic) 1e-4 (seen: about 5e-12 to 5e-11 and 2e-6 to 5e-6).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import composite
+from torchff.workloads import make_workload, differentiation_inputs
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["coords", "charges", "sigma", "epsilon"]
+
+
+def _rms_rel(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+@pytest.fixture(scope="module")
+def case():
+    w = make_workload("water900")
+    q, s, e, T = differentiation_inputs(w)
+    pairs = composite.pair_list(w, w["cutoff"] + w.get("skin", 0.0))
+    return w, q, s, e, T, pairs
+
+
+def _module(w, precision):
+    from torchff.fused import for_workload
+    return for_workload(w, precision)
+
+
+def _inputs(w, q, s, e, dev="cuda"):
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev, requires_grad=True)
+    return t(w["coords"]), torch.tensor(w["box"], dtype=torch.float64, device=dev), t(q), t(s), t(e)
+
+
+def _grads(inp):
+    c, _, q, s, e = inp
+    return [x.grad.double().cpu().numpy() for x in (c, q, s, e)]
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-4)])
+def test_force_matching_gradients(case, precision, tol):
+    w, q, s, e, T, pairs = case
+    L_ref, F_ref, g_ref = composite.force_matching(w, q, s, e, T, pairs=pairs)
+    m = _module(w, precision)
+    inp = _inputs(w, q, s, e)
+    c, box, qt, st, et = inp
+    E, F, _ = m(c, box, qt, st, et)
+    L = torch.mean((F - torch.tensor(T, device="cuda")) ** 2)
+    L.backward()
+    m.check()
+    got = _grads(inp)
+    errs = {k: _rms_rel(a, g_ref[k]) for k, a in zip(NAMES, got)}
+    print(f"{precision}: L={float(L):.10g} ref={L_ref:.10g} grad rms rel {errs}")
+    assert abs(float(L) - L_ref) <= max(tol, 1e-10) * abs(L_ref)
+    for k in NAMES:
+        assert errs[k] < tol, (k, errs[k])
+
+
+def test_energy_gradients(case):
+    # loss = E: only the first-derivative (gE) terms
+    w, q, s, e, T, pairs = case
+    E_ref, g_ref = composite.energy_gradients(w, q, s, e, pairs=pairs)
+    m = _module(w, "fp64")
+    inp = _inputs(w, q, s, e)
+    c, box, qt, st, et = inp
+    E, F, _ = m(c, box, qt, st, et)
+    E.backward()
+    got = _grads(inp)
+    assert abs(float(E) - E_ref) < 1e-9 * abs(E_ref)
+    for k, a in zip(NAMES, got):
+        assert _rms_rel(a, g_ref[k]) < 1e-9, (k, _rms_rel(a, g_ref[k]))
+
+
+def test_mixed_loss_energy_and_forces(case):
+    # L = 0.3 E + mean((F - T)^2): both upstream gradients at once
+    w, q, s, e, T, pairs = case
+    L_ref, _, g_fm = composite.force_matching(w, q, s, e, T, pairs=pairs)
+    E_ref, g_e = composite.energy_gradients(w, q, s, e, pairs=pairs)
+    m = _module(w, "fp64")
+    inp = _inputs(w, q, s, e)
+    c, box, qt, st, et = inp
+    E, F, _ = m(c, box, qt, st, et)
+    L = 0.3 * E + torch.mean((F - torch.tensor(T, device="cuda")) ** 2)
+    L.backward()
+    for k, a in zip(NAMES, _grads(inp)):
+        ref = 0.3 * g_e[k] + g_fm[k]
+        assert _rms_rel(a, ref) < 1e-8, k
+
+
+@pytest.mark.parametrize("which,idx", [("charges", 3), ("sigma", 0), ("epsilon", 6), ("coords", (7, 1))])
+def test_gpu_gradients_match_finite_differences(case, which, idx):
+    # the GPU float64 path against central differences of its own loss (h = 1e-5)
+    w, q, s, e, T, pairs = case
+    m = _module(w, "fp64")
+    Tt = torch.tensor(T, device="cuda")
+    base = {"coords": w["coords"], "charges": q, "sigma": s, "epsilon": e}
+
+    def loss(vals):
+        with torch.no_grad():
+            c, box, qt, st, et = _inputs(w, vals["charges"], vals["sigma"], vals["epsilon"])
+            c = torch.tensor(vals["coords"], dtype=torch.float64, device="cuda")
+            _, F, _ = m(c, box, qt.detach(), st.detach(), et.detach())
+            return float(torch.mean((F - Tt) ** 2))
+
+    inp = _inputs(w, q, s, e)
+    c, box, qt, st, et = inp
+    _, F, _ = m(c, box, qt, st, et)
+    torch.mean((F - Tt) ** 2).backward()
+    g = dict(zip(NAMES, _grads(inp)))
+    h = 1e-5
+    plus = {k: np.array(v, copy=True) for k, v in base.items()}
+    minus = {k: np.array(v, copy=True) for k, v in base.items()}
+    plus[which][idx] += h
+    minus[which][idx] -= h
+    fd = (loss(plus) - loss(minus)) / (2 * h)
+    assert abs(fd - g[which][idx]) <= 1e-5 * max(1.0, abs(fd)), (which, idx, fd, g[which][idx])

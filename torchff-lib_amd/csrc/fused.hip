@@ -882,6 +882,203 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_f64(const int4* __restrict__
   tiles_body<double>(cpairs, ncp_ptr, cap, xyzq, lj, excl_part, ta, facc, eacc);
 }
 
+// ---------------------------------------------------------------- 4b. force-matching backward
+// For upstream gradients gE (of E) and v (of F = -dE/dr), the vector-Jacobian product of (E, F):
+//   dL/dtheta = gE dE/dtheta - d(D_v E)/dtheta,   dL/dr = gE dE/dr - grad(D_v E),
+// with D_v E = sum_i v_i . grad_i E.  For a pair (d = r_i - r_j, dv = v_i - v_j, s = d . dv) with
+// energy phi(r), f = phi'/r and h = f'/r:  D_v E_pair = f s,  grad_i(f s) = f dv + s h d.
+//   Coulomb  phi = k q_i q_j c/r, c = erfc(a r) - [excluded],  g = c + (2/sqrt pi) a r e^{-a^2 r^2}:
+//            f = -k q_i q_j g / r^3,  h = k q_i q_j (3 g / r^5 + (4/sqrt pi) a^3 e^{-a^2 r^2} / r^2)
+//   LJ       phi = 4 eps (s12 - s6):  f = -4 eps (12 s12 - 6 s6) / r^2,
+//            h = 4 eps (168 s12 - 48 s6) / r^4;  d/dsigma_i = 1/2 d/dsigma_ij,
+//            d/deps_i = (.)/(2 eps_i) (0 where eps_i = 0).
+// Per slot six int64 accumulators: d/dr (3), d/dq, d/dsigma, d/deps.
+constexpr int GN = 6;
+
+template <typename TI, typename TC>
+__global__ __launch_bounds__(FB) void k_fill_vslots(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
+                                                    const TI* __restrict__ v, V4<TC>* __restrict__ vsl) {
+  const int sl = blockIdx.x * FB + threadIdx.x;
+  if (sl >= *ncl_ptr * CL) return;
+  const int i = slot_orig[sl];
+  vsl[sl] = (i >= 0 && v) ? V4<TC>{(TC)v[3 * i], (TC)v[3 * i + 1], (TC)v[3 * i + 2], (TC)0}
+                          : V4<TC>{(TC)0, (TC)0, (TC)0, (TC)0};
+}
+
+// erfc(u), exp(-u^2) in the accuracy of the path (float: the fitted form above)
+template <typename TC>
+__device__ __forceinline__ void pair_grad_terms(TC r2, bool excl, bool lj_on, TC kqi, TC qi, TC qj, TC si, TC sj,
+                                                TC ei2, TC ej2, const TileArgs<TC>& ta, TC& f, TC& h, TC& phi_c,
+                                                TC& dpqi, TC& dpqj, TC& dfqi, TC& dfqj, TC& phi_lj, TC& f_lj,
+                                                TC& dps, TC& dfs) {
+  const TC c2 = (TC)1.1283791670955126;  // 2 / sqrt(pi)
+  const TC rinv = rsqrt_t(r2);
+  const TC rinv2 = rinv * rinv;
+  const TC u = ta.alpha * (r2 * rinv);
+  TC ec, em;
+  erfc_exp(u, ec, em);
+  if (excl) ec -= (TC)1;
+  const TC g = fma(c2 * u, em, ec);
+  const TC rinv3 = rinv * rinv2;
+  // Coulomb (ta.coul is folded into kqi = k q_i; q_i, q_j plain)
+  phi_c = kqi * qj * ec * rinv;
+  const TC fc = -kqi * qj * g * rinv3;
+  const TC a3 = ta.alpha * ta.alpha * ta.alpha;
+  const TC hc = kqi * qj * ((TC)3 * g * rinv3 * rinv2 + (TC)2 * c2 * a3 * em * rinv2);
+  dpqi = ta.coul * qj * ec * rinv;
+  dpqj = kqi * ec * rinv;
+  dfqi = -ta.coul * qj * g * rinv3;
+  dfqj = -kqi * g * rinv3;
+  // Lennard-Jones (sigma / 2 and 2 sqrt(eps) per atom)
+  TC eps4 = ei2 * ej2;
+  if (!lj_on) eps4 = (TC)0;
+  const TC sg = si + sj;
+  const TC s2 = sg * sg * rinv2;
+  const TC s6 = s2 * s2 * s2;
+  const TC s12 = s6 * s6;
+  phi_lj = eps4 * (s12 - s6);
+  f_lj = -eps4 * ((TC)12 * s12 - (TC)6 * s6) * rinv2;
+  const TC hl = eps4 * ((TC)168 * s12 - (TC)48 * s6) * rinv2 * rinv2;
+  const TC sginv = sg > (TC)0 ? (TC)1 / sg : (TC)0;
+  dps = (TC)0.5 * eps4 * ((TC)12 * s12 - (TC)6 * s6) * sginv;
+  dfs = (TC)-0.5 * eps4 * ((TC)144 * s12 - (TC)36 * s6) * rinv2 * sginv;
+  f = fc + f_lj;
+  h = hc + hl;
+}
+
+// One wave per workgroup over a contiguous run of the tile list (as the forward pair kernel);
+// one i-group at a time (lane = (ii, jj) holds i-slot gi * 8 + ii against j-slot gj * 8 + jj),
+// minimum image per pair.
+template <typename TC>
+__global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict__ cpairs,
+                                                         const int* __restrict__ ncp_ptr, long long cap,
+                                                         const V4<TC>* __restrict__ xyzq,
+                                                         const V2<TC>* __restrict__ lj,
+                                                         const V4<TC>* __restrict__ vsl,
+                                                         const int* __restrict__ excl_part, TileArgs<TC> ta,
+                                                         const double* __restrict__ gE_ptr,
+                                                         unsigned long long* __restrict__ gacc) {
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const int ii = lane & 7, jj = lane >> 3;
+  long long ncp = *ncp_ptr;
+  if (ncp > cap) ncp = cap;
+  const long long nw = gridDim.x, gw = blockIdx.x;
+  const long long per = (ncp + nw - 1) / nw;
+  const long long t0 = gw * per;
+  const long long t1 = t0 + per < ncp ? t0 + per : ncp;
+  const TC gE = gE_ptr ? (TC)gE_ptr[0] : (TC)0;
+  for (long long t = t0; t < t1; ++t) {
+    const int4 cp = cpairs[t];
+    const int ci = cp.x, cj = cp.y;
+    const unsigned info = (unsigned)cp.z;
+    const unsigned mask = info & 0xffffu;
+    if (mask == 0) continue;
+    const bool upper = ci != cj || jj > ii;
+    V4<TC> pj[4], vj[4];
+    V2<TC> lJ[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int sl = cj * CL + g * 8 + jj;
+      pj[g] = xyzq[sl];
+      lJ[g] = lj[sl];
+      vj[g] = vsl[sl];
+    }
+    TC aj[4][GN];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int k = 0; k < GN; ++k) aj[g][k] = 0;
+#pragma unroll
+    for (int gi = 0; gi < 4; ++gi) {
+      if (!((mask >> (gi * 4)) & 0xfu)) continue;
+      const int sli = ci * CL + gi * 8 + ii;
+      const V4<TC> pi = xyzq[sli];
+      const V2<TC> li = lj[sli];
+      const V4<TC> vi = vsl[sli];
+      unsigned exm = 0u;
+      if (info & INFO_EXCL) {
+#pragma unroll
+        for (int k = 0; k < MAXEX; ++k) {
+          const int pv = excl_part[sli * MAXEX + k];
+          if (pv >= 0 && (pv >> 5) == cj) exm |= 1u << (pv & 31);
+        }
+      }
+      TC ai[GN];
+#pragma unroll
+      for (int k = 0; k < GN; ++k) ai[k] = 0;
+      const TC kqi = ta.coul * pi.w;
+      const TC ei_inv2 = li.y > (TC)0 ? (TC)2 / (li.y * li.y) : (TC)0;  // 1 / (2 eps_i)
+#pragma unroll
+      for (int gj = 0; gj < 4; ++gj) {
+        if (!((mask >> (gi * 4 + gj)) & 1u)) continue;
+        TC dx = pi.x - pj[gj].x, dy = pi.y - pj[gj].y, dz = pi.z - pj[gj].z;
+        dx -= ta.L[0] * rint(dx * ta.invL[0]);
+        dy -= ta.L[1] * rint(dy * ta.invL[1]);
+        dz -= ta.L[2] * rint(dz * ta.invL[2]);
+        TC r2 = dx * dx + dy * dy + dz * dz;
+        bool ok = r2 <= ta.rc2 && pi.x < (TC)PAD_LIMIT && pj[gj].x < (TC)PAD_LIMIT;
+        if (gi == gj) ok = ok && upper;
+        const bool excl = (exm >> (gj * 8 + jj)) & 1u;
+        r2 = ok ? r2 : (TC)1e30;
+        TC f, h, phi_c, dpqi, dpqj, dfqi, dfqj, phi_lj, f_lj, dps, dfs;
+        pair_grad_terms<TC>(r2, excl, !excl, kqi, pi.w, pj[gj].w, li.x, lJ[gj].x, li.y, lJ[gj].y, ta, f, h, phi_c,
+                            dpqi, dpqj, dfqi, dfqj, phi_lj, f_lj, dps, dfs);
+        const TC okf = ok ? (TC)1 : (TC)0;
+        const TC dvx = vi.x - vj[gj].x, dvy = vi.y - vj[gj].y, dvz = vi.z - vj[gj].z;
+        const TC sdv = dx * dvx + dy * dvy + dz * dvz;
+        const TC cd = okf * (gE * f - sdv * h), cv = okf * f;
+        const TC gx = cd * dx - cv * dvx, gy = cd * dy - cv * dvy, gz = cd * dz - cv * dvz;
+        ai[0] += gx;
+        ai[1] += gy;
+        ai[2] += gz;
+        aj[gj][0] -= gx;
+        aj[gj][1] -= gy;
+        aj[gj][2] -= gz;
+        ai[3] += okf * (gE * dpqi - sdv * dfqi);
+        aj[gj][3] += okf * (gE * dpqj - sdv * dfqj);
+        const TC ts = okf * (gE * dps - sdv * dfs);
+        ai[4] += ts;
+        aj[gj][4] += ts;
+        const TC te = okf * (gE * phi_lj - sdv * f_lj);
+        const TC ej_inv2 = lJ[gj].y > (TC)0 ? (TC)2 / (lJ[gj].y * lJ[gj].y) : (TC)0;
+        ai[5] += te * ei_inv2;
+        aj[gj][5] += te * ej_inv2;
+        (void)phi_c;
+      }
+      // i-slot sums over the 8 lanes sharing ii; lane jj < GN writes value jj
+      TC mine = 0;
+#pragma unroll
+      for (int k = 0; k < GN; ++k) {
+        const TC sk = stride8_sum(ai[k]);
+        mine = jj == k ? sk : mine;
+      }
+      if (jj < GN && mine != (TC)0) atomicAdd(&gacc[(long long)GN * sli + jj], (unsigned long long)to_fx((double)mine, F_SCALE));
+    }
+    // j-slot sums over the 8 lanes sharing jj; lane ii writes values ii, ii + 8, ii + 16 (< 24)
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int k = 0; k < GN; ++k) aj[g][k] = octet_sum(aj[g][k]);
+    TC w0 = 0, w1 = 0, w2 = 0;
+#pragma unroll
+    for (int q = 0; q < 4 * GN; ++q) {
+      const TC vq = aj[q / GN][q % GN];
+      w0 = ii == q ? vq : w0;
+      w1 = ii + 8 == q ? vq : w1;
+      w2 = ii + 16 == q ? vq : w2;
+    }
+    const TC wv[3] = {w0, w1, w2};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int q = ii + 8 * r;
+      if (q < 4 * GN && wv[r] != (TC)0) {
+        const int slj = cj * CL + (q / GN) * 8 + jj;
+        atomicAdd(&gacc[(long long)GN * slj + q % GN], (unsigned long long)to_fx((double)wv[r], F_SCALE));
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- 5. PME on cluster slots
 // Owner-computes spreading: one workgroup per BR^3 block ("brick") of grid points.  The
 // workgroup lists the clusters whose stencils can reach its brick (column neighbourhood,
@@ -927,15 +1124,19 @@ __device__ __forceinline__ void stencil_interval(float slo, float shi, int P, in
   len = hi - lo + 1;
 }
 
-template <typename TC, int P>
+// DIPOLE: instead of the charges, spread the dipoles q_i v_i . grad_r theta_i(m) (the grid of the
+// force-matching backward), from per-slot vectors vsl and the Cartesian-to-grid scale jsc = K / L.
+template <typename TC, int P, bool DIPOLE>
 __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
                                                        const int* __restrict__ slot_orig, const float4* __restrict__ bb_c,
                                                        const float4* __restrict__ bb_h,
                                                        const int* __restrict__ col_cl_start, Geo g, PmeGeom pg,
-                                                       unsigned long long* __restrict__ grid, int* __restrict__ status) {
+                                                       unsigned long long* __restrict__ grid, int* __restrict__ status,
+                                                       const V4<TC>* __restrict__ vsl, V4<TC> jsc) {
   __shared__ unsigned long long s_q[BR * BR * BR];
   __shared__ int s_list[BRICK_MAXC];
   __shared__ V4<TC> s_pos[SPREAD_CHUNK * CL];
+  __shared__ int s_slot[DIPOLE ? SPREAD_CHUNK * CL : 1];
   __shared__ int s_coff[SPREAD_MAXCOL + 1], s_ccs[SPREAD_MAXCOL];
   __shared__ int s_n, s_na;
   const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
@@ -1030,7 +1231,11 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
         anchor_axis(sa, pg.K[a], P, base, t);
         ov = ov && circ_overlap(base, P, B0[a], Bn[a], pg.K[a]);
       }
-      if (ov) s_pos[atomicAdd(&s_na, 1)] = p;
+      if (ov) {
+        const int idx = atomicAdd(&s_na, 1);
+        s_pos[idx] = p;
+        if (DIPOLE) s_slot[idx] = sl;
+      }
     }
     __syncthreads();
     const int na = s_na;
@@ -1056,6 +1261,14 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
         inmask[a] = m;
       }
       const TC qi = p.w;
+      TC wv[3] = {(TC)0, (TC)0, (TC)0};
+      if (DIPOLE) {
+        const V4<TC> vv = vsl[s_slot[k]];
+        // d theta / d r_a = -K_a / L_a M'_a ... (the weights fall as the scaled coordinate grows)
+        wv[0] = -qi * vv.x * jsc.x;
+        wv[1] = -qi * vv.y * jsc.y;
+        wv[2] = -qi * vv.z * jsc.z;
+      }
 #pragma unroll
       for (int ka = 0; ka < P; ++ka) {
         if (!((inmask[0] >> ka) & 1u)) continue;
@@ -1068,13 +1281,19 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
           int gy = base[1] + kb;
           gy = gy >= K1 ? gy - K1 : gy;
           const TC wab = wa * Mv[1][kb];
+          // dipole: value(kc) = cz * M_z(kc) + dz * M_z'(kc)
+          const TC cz = DIPOLE ? wv[0] * dM[0][ka] * Mv[1][kb] + wv[1] * Mv[0][ka] * dM[1][kb] : (TC)0;
+          const TC dz = DIPOLE ? wv[2] * Mv[0][ka] * Mv[1][kb] : (TC)0;
           unsigned long long* row = s_q + ((gx - B0[0]) * BR + (gy - B0[1])) * BR;
 #pragma unroll
           for (int kc = 0; kc < P; ++kc) {
             if (!((inmask[2] >> kc) & 1u)) continue;
             int gz = base[2] + kc;
             gz = gz >= K2 ? gz - K2 : gz;
-            atomicAdd(&row[gz - B0[2]], (unsigned long long)grid_fx(wab * Mv[2][kc]));
+            if (DIPOLE)
+              atomicAdd(&row[gz - B0[2]], (unsigned long long)to_fx((double)(cz * Mv[2][kc] + dz * dM[2][kc]), Q_SCALE));
+            else
+              atomicAdd(&row[gz - B0[2]], (unsigned long long)grid_fx(wab * Mv[2][kc]));
           }
         }
       }
@@ -1146,6 +1365,128 @@ __global__ __launch_bounds__(FB) void k_gather_slots(const int* __restrict__ ncl
   facc[3 * sl + 0] = (unsigned long long)to_fx(-kq * gxr, F_SCALE);
   facc[3 * sl + 1] = (unsigned long long)to_fx(-kq * gyr, F_SCALE);
   facc[3 * sl + 2] = (unsigned long long)to_fx(-kq * gzr, F_SCALE);
+}
+
+// Backward gather, one thread per slot: from the potential grid Phi (of the charges) and Psi (of
+// the dipoles q v . grad theta), the reciprocal-space parts
+//   dL/dq_i = k [gE phi(r_i) - v_i . grad phi(r_i) - psi(r_i)],
+//   dL/dr_i = k q_i [gE grad phi(r_i) - H_phi(r_i) v_i - grad psi(r_i)]
+// (orthorhombic box: d s_a / d r_a = K_a / L_a), as int64 fixed point into gpme (4 per slot).
+template <typename TC, int P>
+__global__ __launch_bounds__(FB) void k_gather_grad(const int* __restrict__ ncl_ptr, const TC* __restrict__ phi_grid,
+                                                    const TC* __restrict__ psi_grid, const V4<TC>* __restrict__ xyzq,
+                                                    const V4<TC>* __restrict__ vsl, const int* __restrict__ slot_orig,
+                                                    PmeGeom g, double coul, const double* __restrict__ gE_ptr,
+                                                    unsigned long long* __restrict__ gpme) {
+  const int sl = blockIdx.x * FB + threadIdx.x;
+  if (sl >= *ncl_ptr * CL || slot_orig[sl] < 0) return;
+  const V4<TC> p = xyzq[sl];
+  const V4<TC> v = vsl[sl];
+  TC M[3][P], D1[3][P], D2[3][P];
+  int base[3];
+  const double r[3] = {(double)p.x, (double)p.y, (double)p.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double sa = (r[0] * g.inv[a] + r[1] * g.inv[3 + a] + r[2] * g.inv[6 + a]) * (double)g.K[a];
+    double t;
+    anchor_axis(sa, g.K[a], P, base[a], t);
+    spline_weights2<TC, P>((TC)t, M[a], D1[a], D2[a]);
+  }
+  const int K1 = g.K[1], K2 = g.K[2];
+  // separable sums over z, then y, then x (derivatives in t; d/ds = -d/dt)
+  double phi = 0, px = 0, py = 0, pz = 0, hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0, hyz = 0;
+  double psi = 0, sx = 0, sy = 0, sz = 0;
+  int gx = base[0];
+#pragma unroll 1
+  for (int a = 0; a < P; ++a) {
+    double ty0 = 0, ty1 = 0, ty2 = 0, tz1 = 0, tz2 = 0, tyz = 0, uy0 = 0, uy1 = 0, uz1 = 0;
+    int gy = base[1];
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const long long row = ((long long)gx * K1 + gy) * K2;
+      double z0 = 0, z1 = 0, z2 = 0, w0 = 0, w1 = 0;
+      int gz = base[2];
+#pragma unroll
+      for (int c = 0; c < P; ++c) {
+        const double f = (double)phi_grid[row + gz], q = (double)psi_grid[row + gz];
+        z0 += (double)M[2][c] * f;
+        z1 += (double)D1[2][c] * f;
+        z2 += (double)D2[2][c] * f;
+        w0 += (double)M[2][c] * q;
+        w1 += (double)D1[2][c] * q;
+        gz = gz + 1 == K2 ? 0 : gz + 1;
+      }
+      const double my = M[1][b], dy = D1[1][b], ddy = D2[1][b];
+      ty0 += my * z0;
+      ty1 += dy * z0;
+      ty2 += ddy * z0;
+      tz1 += my * z1;
+      tz2 += my * z2;
+      tyz += dy * z1;
+      uy0 += my * w0;
+      uy1 += dy * w0;
+      uz1 += my * w1;
+      gy = gy + 1 == K1 ? 0 : gy + 1;
+    }
+    const double mx = M[0][a], dx = D1[0][a], ddx = D2[0][a];
+    phi += mx * ty0;
+    px += dx * ty0;
+    py += mx * ty1;
+    pz += mx * tz1;
+    hxx += ddx * ty0;
+    hyy += mx * ty2;
+    hzz += mx * tz2;
+    hxy += dx * ty1;
+    hxz += dx * tz1;
+    hyz += mx * tyz;
+    psi += mx * uy0;
+    sx += dx * uy0;
+    sy += mx * uy1;
+    sz += mx * uz1;
+    gx = gx + 1 == g.K[0] ? 0 : gx + 1;
+  }
+  const double j0 = g.K[0] * g.inv[0], j1 = g.K[1] * g.inv[4], j2 = g.K[2] * g.inv[8];
+  // Cartesian gradient (first derivatives change sign from t to s) and Hessian
+  const double gp[3] = {-px * j0, -py * j1, -pz * j2};
+  const double gs[3] = {-sx * j0, -sy * j1, -sz * j2};
+  const double H[3][3] = {{hxx * j0 * j0, hxy * j0 * j1, hxz * j0 * j2},
+                          {hxy * j0 * j1, hyy * j1 * j1, hyz * j1 * j2},
+                          {hxz * j0 * j2, hyz * j1 * j2, hzz * j2 * j2}};
+  const double gE = gE_ptr ? gE_ptr[0] : 0.0;
+  const double vv[3] = {(double)v.x, (double)v.y, (double)v.z};
+  const double q = (double)p.w;
+  const double dq = coul * (gE * phi - (vv[0] * gp[0] + vv[1] * gp[1] + vv[2] * gp[2]) - psi);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double hv = H[a][0] * vv[0] + H[a][1] * vv[1] + H[a][2] * vv[2];
+    gpme[4 * (long long)sl + a] = (unsigned long long)to_fx(coul * q * (gE * gp[a] - hv - gs[a]), F_SCALE);
+  }
+  gpme[4 * (long long)sl + 3] = (unsigned long long)to_fx(dq, F_SCALE);
+}
+
+// gradients back in the caller's atom order (pair + reciprocal accumulators; self energy
+// -k alpha / sqrt(pi) sum q^2 contributes gE * (-2 k alpha / sqrt(pi)) q_i to dL/dq_i)
+template <typename TI>
+__global__ __launch_bounds__(FB) void k_finalize_grad(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
+                                                      const unsigned long long* __restrict__ gacc,
+                                                      const unsigned long long* __restrict__ gpme,
+                                                      const TI* __restrict__ q, double coul, double alpha,
+                                                      const double* __restrict__ gE_ptr, TI* __restrict__ dr,
+                                                      TI* __restrict__ dq, TI* __restrict__ dsig, TI* __restrict__ deps) {
+  const int sl = blockIdx.x * FB + threadIdx.x;
+  if (sl >= *ncl_ptr * CL) return;
+  const int i = slot_orig[sl];
+  if (i < 0) return;
+  const long long b6 = (long long)GN * sl, b4 = 4 * (long long)sl;
+  if (dr)
+    for (int a = 0; a < 3; ++a) dr[3 * i + a] = (TI)((double)(long long)(gacc[b6 + a] + gpme[b4 + a]) / F_SCALE);
+  if (dq) {
+    const double gE = gE_ptr ? gE_ptr[0] : 0.0;
+    dq[i] = (TI)((double)(long long)(gacc[b6 + 3] + gpme[b4 + 3]) / F_SCALE -
+                 gE * 2.0 * coul * alpha * 0.5641895835477563 * (double)q[i]);
+  }
+  if (dsig) dsig[i] = (TI)((double)(long long)gacc[b6 + 4] / F_SCALE);
+  if (deps) deps[i] = (TI)((double)(long long)gacc[b6 + 5] / F_SCALE);
 }
 
 // ---------------------------------------------------------------- 6. finalize
@@ -1241,11 +1582,14 @@ struct FusedWs {
   void* work;
   double *kparts, *qparts;
   unsigned long long* eacc;
+  // backward only
+  void* vsl;
+  unsigned long long *gacc, *gpme, *grid_d;
   size_t bytes;
   int nk, nw_tiles, nq;
 };
 
-FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) {
+FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc, bool grad = false) {
   FusedWs w{};
   size_t off = 0;
   char* p = (char*)base;
@@ -1291,6 +1635,12 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc) 
   w.kparts = (double*)take(8 * (size_t)w.nk);
   w.nq = (int)((nn + FB - 1) / FB);
   w.qparts = (double*)take(8 * (size_t)w.nq);
+  if (grad) {
+    w.vsl = take(4 * tc * slots);
+    w.gacc = (unsigned long long*)take(8 * GN * slots);
+    w.gpme = (unsigned long long*)take(8 * 4 * slots);
+    w.grid_d = (unsigned long long*)take(8 * ng);
+  }
   w.bytes = off;
   return w;
 }
@@ -1302,8 +1652,9 @@ int launch_pme_slots(int order, const FusedWs& w, const Geo& g, const PmeGeom& p
   const unsigned nbricks = (unsigned)(((pg.K[0] + BR - 1) / BR) * ((pg.K[1] + BR - 1) / BR) * ((pg.K[2] + BR - 1) / BR));
 #define TFF_SP(P)                                                                                               \
   if (spread)                                                                                                   \
-    hipLaunchKernelGGL((k_spread_bricks<TC, P>), dim3(nbricks), dim3(256), 0, st, w.ncl, (const V4<TC>*)w.xyzq, \
-                       w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status);                      \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P, false>), dim3(nbricks), dim3(256), 0, st, w.ncl,               \
+                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status, \
+                       (const V4<TC>*)nullptr, V4<TC>{});                                                        \
   else                                                                                                          \
     hipLaunchKernelGGL((k_gather_slots<TC, P>), dim3(nb), dim3(FB), 0, st, w.ncl, phi, (const V4<TC>*)w.xyzq,   \
                        w.slot_orig, pg, coul, w.facc_pme)
@@ -1353,6 +1704,56 @@ void aux_stream_join(hipStream_t st) {
   AuxStream& a = g_aux[dev];
   hipEventRecord(a.join, a.s);
   hipStreamWaitEvent(st, a.join, 0);
+}
+
+// Reciprocal-space convolution of a fixed-point grid: forward transforms, influence function with
+// the k-space energy partials, inverse transforms; the real potential grid phi_out may alias
+// grid_in (it is written by the last pass only).
+template <typename TC>
+void run_recip_fft(const FusedWs& w, const int K[3], const TffBox& box, double alpha, const FftPlan* pl,
+                   const TC* mx, const TC* my, const TC* mz, const unsigned long long* grid_in,
+                   double grid_inv_scale, TC* phi_out, double* kparts, const char* prof_name, hipStream_t st) {
+  ConvArgs cv;
+  for (int a = 0; a < 3; ++a) cv.K[a] = K[a];
+  for (int k = 0; k < 9; ++k) cv.inv[k] = box.inv[k];
+  cv.vol = box.det;
+  cv.alpha = alpha;
+  cv.grid_fixed = 1;
+  cv.grid_inv_scale = grid_inv_scale;
+  const size_t ce = 2 * sizeof(TC);
+  // real-input transforms along z when K2 is even (half spectrum, K2 / 2 + 1 planes), else complex
+  const bool rz = K[2] % 2 == 0;
+  const int nzc = spectrum_nz(K);
+  cv.nz = nzc;
+  cv.half = rz ? 1 : 0;
+  PassCfg cz = pass_cfg(K[0] * K[1], K[2], 1, ce);
+  PassCfg cy = pass_cfg(K[0], K[1], nzc, ce);
+  PassCfg cxp = pass_cfg(1, K[0], K[1] * nzc, ce);
+  FftPlan pzh;
+  make_plan(rz ? K[2] / 2 : 1, pzh);
+  const ZPlanCfg zc = zcfg(K[0] * K[1], K[2], ce);
+  {
+    void* tok = tff_prof_begin(prof_name, st);
+    if (rz)
+      hipLaunchKernelGGL(k_fft_z_r2c<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, st, grid_in, (cx<TC>*)w.work,
+                         K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh, 1, cv.grid_inv_scale);
+    else
+      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_FWD>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, grid_in, w.work,
+                         cz.pa, pl[2], cv, mx, my, mz, 1, kparts);
+    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_FWD>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, st, w.work, w.work,
+                       cy.pa, pl[1], cv, mx, my, mz, 1, kparts);
+    hipLaunchKernelGGL((k_fft_pass<TC, PASS_X_CONV>), dim3(cxp.tiles), dim3(FFT_THREADS), cxp.lds, st, w.work,
+                       w.work, cxp.pa, pl[0], cv, mx, my, mz, 1, kparts);
+    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_INV>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, st, w.work, w.work,
+                       cy.pa, pl[1], cv, mx, my, mz, 1, kparts);
+    if (rz)
+      hipLaunchKernelGGL(k_fft_z_c2r<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, st, (const cx<TC>*)w.work, phi_out,
+                         K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh);
+    else
+      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_INV>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, st, w.work, phi_out,
+                         cz.pa, pl[2], cv, mx, my, mz, 1, kparts);
+    tff_prof_end(tok, st);
+  }
 }
 
 template <typename TI, typename TC>
@@ -1413,48 +1814,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     tff_prof_end(tok, sp);
     if (rc) return rc;
   }
-  ConvArgs cv;
-  for (int a = 0; a < 3; ++a) cv.K[a] = K[a];
-  for (int k = 0; k < 9; ++k) cv.inv[k] = box.inv[k];
-  cv.vol = box.det;
-  cv.alpha = alpha;
-  cv.grid_fixed = 1;
-  cv.grid_inv_scale = 1.0 / grid_scale<TC>();
-  const size_t ce = 2 * sizeof(TC);
-  // real-input transforms along z when K2 is even (half spectrum, K2 / 2 + 1 planes), else complex
-  const bool rz = K[2] % 2 == 0;
-  const int nzc = spectrum_nz(K);
-  cv.nz = nzc;
-  cv.half = rz ? 1 : 0;
-  PassCfg cz = pass_cfg(K[0] * K[1], K[2], 1, ce);
-  PassCfg cy = pass_cfg(K[0], K[1], nzc, ce);
-  PassCfg cxp = pass_cfg(1, K[0], K[1] * nzc, ce);
-  FftPlan pzh;
-  make_plan(rz ? K[2] / 2 : 1, pzh);
-  const ZPlanCfg zc = zcfg(K[0] * K[1], K[2], ce);
   TC* phi = (TC*)w.grid;  // the real potential grid reuses the charge grid's memory after the z pass
-  {
-    void* tok = tff_prof_begin("fused_fft", sp);
-    if (rz)
-      hipLaunchKernelGGL(k_fft_z_r2c<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, sp, w.grid, (cx<TC>*)w.work,
-                         K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh, 1, cv.grid_inv_scale);
-    else
-      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_FWD>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, sp, w.grid, w.work,
-                         cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_FWD>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, sp, w.work, w.work,
-                       cy.pa, pl[1], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_X_CONV>), dim3(cxp.tiles), dim3(FFT_THREADS), cxp.lds, sp, w.work,
-                       w.work, cxp.pa, pl[0], cv, mx, my, mz, 1, w.kparts);
-    hipLaunchKernelGGL((k_fft_pass<TC, PASS_Y_INV>), dim3(cy.tiles), dim3(FFT_THREADS), cy.lds, sp, w.work, w.work,
-                       cy.pa, pl[1], cv, mx, my, mz, 1, w.kparts);
-    if (rz)
-      hipLaunchKernelGGL(k_fft_z_c2r<TC>, dim3(zc.blocks), dim3(FFT_THREADS), zc.lds, sp, (const cx<TC>*)w.work, phi,
-                         K[0] * K[1], K[2], zc.AB, zc.ab_shift, pzh);
-    else
-      hipLaunchKernelGGL((k_fft_pass<TC, PASS_Z_INV>), dim3(cz.tiles), dim3(FFT_THREADS), cz.lds, sp, w.work, phi,
-                         cz.pa, pl[2], cv, mx, my, mz, 1, w.kparts);
-    tff_prof_end(tok, sp);
-  }
+  run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts, "fused_fft", sp);
   {
     void* tok = tff_prof_begin("fused_gather", sp);
     int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, phi, coul, false, status, sp);
@@ -1503,6 +1864,108 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   return TFF_OK;
 }
 
+// Force-matching backward: rebuilds the cluster pair list (not stored from the forward call),
+// then the pair backward, the charge and dipole reciprocal convolutions and the gather.
+template <typename TI, typename TC>
+int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, long long n, const TffBox& box,
+                   const Geo& g, const int64_t* row_ptr, const int64_t* col, double cutoff, double list_cutoff,
+                   double alpha, double coul, const int K[3], int order, const TC* mx, const TC* my, const TC* mz,
+                   const double* gE, const TI* gF, TI* dr, TI* dq, TI* dsig, TI* deps, int* status, const FusedWs& w,
+                   hipStream_t st) {
+  if (order < 3 || order > 8) {
+    tff_set_error("fused: unsupported spline order %d (3..8)", order);
+    return TFF_ERR_UNSUPPORTED;
+  }
+  const unsigned nbA = (unsigned)((n + FB - 1) / FB);
+  const int nslots = g.ncl_max * CL;
+  const unsigned nbS = (unsigned)((nslots + FB - 1) / FB);
+  const int ncols = g.ncol[0] * g.ncol[1];
+  {
+    ClearList cl{};
+    auto add = [&](void* p, long long words) {
+      cl.p[cl.count] = (int*)p;
+      cl.n[cl.count] = words;
+      ++cl.count;
+    };
+    add(w.col_count, ncols + 1);
+    add(w.ncp, 4);
+    add(status, 2);
+    add(w.gacc, 2LL * GN * nslots);
+    add(w.slot_orig, nslots);
+    hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, cl);
+  }
+  hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, q, n, g, w.col_of, w.off_in_col,
+                     w.col_count, w.qparts);
+  hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.col_count, ncols, w.col_start, w.col_cl_start,
+                     w.ncl, status);
+  hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
+                     w.sorted_orig);
+  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(256), 0, st, coords, g, w.col_start, w.sorted_orig);
+  const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
+  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbW), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
+                     w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
+                     w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
+  hipLaunchKernelGGL((k_fill_vslots<TI, TC>), dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, gF, (V4<TC>*)w.vsl);
+  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
+                     col, w.excl_part, status);
+  hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c, w.bb_h,
+                     g, (float)(list_cutoff * list_cutoff), w.excl_part, w.seg, w.seg_count, status);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
+  hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
+                     w.sb_c, w.sb_h, g, (float)(cutoff * cutoff), w.cpairs);
+  {
+    TileArgs<TC> ta;
+    for (int a = 0; a < 3; ++a) {
+      ta.L[a] = (TC)g.L[a];
+      ta.invL[a] = (TC)g.invL[a];
+    }
+    ta.rc2 = (TC)(cutoff * cutoff);
+    ta.alpha = (TC)alpha;
+    ta.coul = (TC)coul;
+    void* tok = tff_prof_begin("fused_grad_tiles", st);
+    hipLaunchKernelGGL(k_tiles_grad<TC>, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
+                       g.pair_cap, (const V4<TC>*)w.xyzq, (const V2<TC>*)w.lj, (const V4<TC>*)w.vsl, w.excl_part, ta,
+                       gE, w.gacc);
+    tff_prof_end(tok, st);
+  }
+  PmeGeom pg = make_geom(box, K, order, alpha);
+  FftPlan pl[3];
+  for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
+  const unsigned nbricks = (unsigned)(((K[0] + BR - 1) / BR) * ((K[1] + BR - 1) / BR) * ((K[2] + BR - 1) / BR));
+  V4<TC> jsc{(TC)(K[0] * box.inv[0]), (TC)(K[1] * box.inv[4]), (TC)(K[2] * box.inv[8]), (TC)0};
+  TC* phi = (TC*)w.grid;
+  TC* psi = (TC*)w.grid_d;
+  const unsigned nb = (unsigned)((nslots + FB - 1) / FB);
+#define TFF_GRAD_PME(P)                                                                                          \
+  {                                                                                                              \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P, false>), dim3(nbricks), dim3(256), 0, st, w.ncl,                  \
+                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status, \
+                       (const V4<TC>*)nullptr, V4<TC>{});                                                        \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P, true>), dim3(nbricks), dim3(256), 0, st, w.ncl,                   \
+                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid_d,       \
+                       status, (const V4<TC>*)w.vsl, jsc);                                                       \
+    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts,          \
+                      "fused_grad_fft", st);                                                                     \
+    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid_d, 1.0 / Q_SCALE, psi, w.kparts,                 \
+                      "fused_grad_fft", st);                                                                     \
+    hipLaunchKernelGGL((k_gather_grad<TC, P>), dim3(nb), dim3(FB), 0, st, w.ncl, phi, psi, (const V4<TC>*)w.xyzq, \
+                       (const V4<TC>*)w.vsl, w.slot_orig, pg, coul, gE, w.gpme);                                 \
+  }
+  switch (order) {
+    case 3: TFF_GRAD_PME(3); break;
+    case 4: TFF_GRAD_PME(4); break;
+    case 5: TFF_GRAD_PME(5); break;
+    case 6: TFF_GRAD_PME(6); break;
+    case 7: TFF_GRAD_PME(7); break;
+    default: TFF_GRAD_PME(8); break;
+  }
+#undef TFF_GRAD_PME
+  hipLaunchKernelGGL(k_finalize_grad<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.gacc, w.gpme, q, coul,
+                     alpha, gE, dr, dq, dsig, deps);
+  TFF_CHECK_LAUNCH("fused energy+forces backward");
+  return TFF_OK;
+}
+
 }  // namespace
 
 extern "C" size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const double* box_host,
@@ -1512,6 +1975,86 @@ extern "C" size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms,
   if (!make_geo(box, natoms, list_cutoff, g)) return 0;
   int K[3] = {K0, K1, K2};
   return carve(nullptr, natoms, g, K, compute_dtype == TFF_F64 ? 8 : 4).bytes;
+}
+
+namespace {
+// checks shared by the forward and backward entry points
+int fused_check(const TffBox& box, long long natoms, double cutoff, double list_cutoff, double ewald_alpha,
+                const int K[3], int order, const int64_t* excl_row_ptr, const int64_t* excl_col, Geo& g) {
+  if (!make_geo(box, natoms, list_cutoff, g)) {
+    tff_set_error("fused: needs an orthorhombic periodic box (diagonal box matrix)");
+    return TFF_ERR_UNSUPPORTED;
+  }
+  if (natoms <= 0 || !(cutoff > 0) || list_cutoff < cutoff || ewald_alpha < 0) {
+    tff_set_error("fused: invalid arguments");
+    return TFF_ERR_ARG;
+  }
+  if ((excl_row_ptr == nullptr) != (excl_col == nullptr)) {
+    tff_set_error("fused: exclusion row_ptr and col must be given together");
+    return TFF_ERR_ARG;
+  }
+  for (int a = 0; a < 3; ++a) {
+    FftPlan p;
+    if (K[a] < order || K[a] > 4096 || !make_plan(K[a], p)) {
+      tff_set_error("fused: PME grid size %d must be 2^a 3^b 5^c and >= the spline order", K[a]);
+      return TFF_ERR_UNSUPPORTED;
+    }
+  }
+  for (int a = 0; a < 3; ++a)
+    if (2.0 * list_cutoff > g.L[a]) {
+      tff_set_error("fused: the box (%.4g nm) must exceed twice the list cutoff (%.4g nm) for the minimum image",
+                    g.L[a], list_cutoff);
+      return TFF_ERR_UNSUPPORTED;
+    }
+  return TFF_OK;
+}
+}  // namespace
+
+extern "C" size_t tff_fused_backward_workspace_bytes(int compute_dtype, long long natoms, const double* box_host,
+                                                     double list_cutoff, int K0, int K1, int K2) {
+  TffBox box = tff_make_box(box_host);
+  Geo g;
+  if (!make_geo(box, natoms, list_cutoff, g)) return 0;
+  int K[3] = {K0, K1, K2};
+  return carve(nullptr, natoms, g, K, compute_dtype == TFF_F64 ? 8 : 4, true).bytes;
+}
+
+extern "C" int tff_fused_energy_forces_backward(
+    int io_dtype, int compute_dtype, const void* coords, const void* charges, const void* sigma, const void* epsilon,
+    long long natoms, const double* box_host, const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff,
+    double list_cutoff, double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
+    const void* xmoduli, const void* ymoduli, const void* zmoduli, const double* grad_energy,
+    const void* grad_forces, void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon,
+    int32_t* status_out, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  TffBox box = tff_make_box(box_host);
+  Geo g;
+  int K[3] = {K0, K1, K2};
+  int rc = fused_check(box, natoms, cutoff, list_cutoff, ewald_alpha, K, order, excl_row_ptr, excl_col, g);
+  if (rc) return rc;
+  if (!coords || !charges || !sigma || !epsilon || !status_out || !xmoduli || !ymoduli || !zmoduli) {
+    tff_set_error("fused backward: invalid arguments");
+    return TFF_ERR_ARG;
+  }
+  size_t tc = compute_dtype == TFF_F64 ? 8 : 4;
+  size_t need = carve(nullptr, natoms, g, K, tc, true).bytes;
+  if (!workspace || workspace_bytes < need) {
+    tff_set_error("fused backward: workspace too small (%zu < %zu)", workspace_bytes, need);
+    return TFF_ERR_WORKSPACE;
+  }
+  FusedWs w = carve(workspace, natoms, g, K, tc, true);
+#define TFF_RUN(TI, TC)                                                                                            \
+  return run_fused_grad<TI, TC>((const TI*)coords, (const TI*)charges, (const TI*)sigma, (const TI*)epsilon,       \
+                                natoms, box, g, excl_row_ptr, excl_col, cutoff, list_cutoff, ewald_alpha,          \
+                                coul_constant, K, order, (const TC*)xmoduli, (const TC*)ymoduli,                  \
+                                (const TC*)zmoduli, grad_energy, (const TI*)grad_forces, (TI*)grad_coords,         \
+                                (TI*)grad_charges, (TI*)grad_sigma, (TI*)grad_epsilon, (int*)status_out, w, stream)
+  if (io_dtype == TFF_F64 && compute_dtype == TFF_F32) TFF_RUN(double, float);
+  if (io_dtype == TFF_F32 && compute_dtype == TFF_F32) TFF_RUN(float, float);
+  if (io_dtype == TFF_F64 && compute_dtype == TFF_F64) TFF_RUN(double, double);
+  if (io_dtype == TFF_F32 && compute_dtype == TFF_F64) TFF_RUN(float, double);
+#undef TFF_RUN
+  tff_set_error("fused backward: unsupported dtype combination");
+  return TFF_ERR_ARG;
 }
 
 extern "C" int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,

@@ -11,6 +11,13 @@ tiles (libtorchff_hip.so, ``tff_fused_energy_forces``).
 
 precision="mixed": float32 arithmetic, int64 fixed-point accumulation of forces
 and energies, float64 k-space and self sums.  precision="fp64": float64 throughout.
+
+Differentiation: when an input requires grad, (E, F) are differentiable with respect to
+coords, charges, sigma and epsilon through a hand-written vector-Jacobian product
+(``tff_fused_energy_forces_backward``): a loss of the forces (force matching) gets
+gE dE/dtheta - d(sum_i v_i . dE/dr_i)/dtheta with v = dL/dF, i.e. the double backward the
+reference computes with create_graph on its pure-PyTorch path.  The backward rebuilds the
+cluster pair list; it is not itself differentiable.
 """
 
 import math
@@ -76,6 +83,14 @@ class FusedNonbondedPME(nn.Module):
     def forward(self, coords, box, charges, sigma, epsilon, check: bool = False):
         """Returns (energy float64 scalar, forces shaped like coords, terms float64 (4,) =
         [Lennard-Jones, real-space Coulomb incl. exclusion correction, reciprocal, self])."""
+        if torch.is_grad_enabled() and any(t.requires_grad for t in (coords, charges, sigma, epsilon)):
+            E, F, terms = _FusedEnergyForces.apply(self, coords, charges, sigma, epsilon, box)
+            if check:
+                self.check()
+            return E, F, terms
+        return self._forward_raw(coords, box, charges, sigma, epsilon, check)
+
+    def _forward_raw(self, coords, box, charges, sigma, epsilon, check=False):
         L.require_cuda(coords, box, charges, sigma, epsilon)
         lib = L.load()
         dev = coords.device
@@ -109,6 +124,36 @@ class FusedNonbondedPME(nn.Module):
             self.check()
         return out[4], forces, out[:4]
 
+    def _backward_raw(self, coords, box, charges, sigma, epsilon, grad_energy, grad_forces, needs):
+        """dL/d(coords, charges, sigma, epsilon) for upstream grad_energy (0-d, or None) and
+        grad_forces ((N, 3), or None); entries of ``needs`` False give None."""
+        lib = L.load()
+        dev = coords.device
+        io = coords.dtype
+        c = coords.detach().contiguous()
+        q, s, e = (t.detach().to(io).contiguous() for t in (charges, sigma, epsilon))
+        bh = L.box_host(box)
+        cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
+        key = (tuple(bh), dev)
+        if getattr(self, "_bws", None) is None or self._bws_key != key:
+            nbytes = lib.tff_fused_backward_workspace_bytes(cd, self.natoms, bh, self.list_cutoff, *self.K)
+            self._bws = L.workspace(nbytes, dev)
+            self._bws_key = key
+        ge = None if grad_energy is None else grad_energy.detach().to(device=dev, dtype=torch.float64).reshape(1)
+        gf = None if grad_forces is None else grad_forces.detach().to(device=dev, dtype=io).contiguous()
+        outs = [torch.empty_like(c) if needs[0] else None] + \
+               [torch.empty(self.natoms, dtype=io, device=dev) if n else None for n in needs[1:]]
+        status = torch.empty(2, dtype=torch.int32, device=dev)
+        rp, col = self.excl_row_ptr, self.excl_col_indices
+        rc = lib.tff_fused_energy_forces_backward(
+            L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms, bh, L.ptr(rp), L.ptr(col),
+            self.cutoff, self.list_cutoff, self.alpha, self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
+            L.ptr(ge), L.ptr(gf), *(L.ptr(o) for o in outs), L.ptr(status), L.ptr(self._bws), self._bws.numel(),
+            L.stream_of(c))
+        L.check(rc, "FusedNonbondedPME backward")
+        self.status = status
+        return outs
+
     def tiles(self):
         """Number of 32 x 32 cluster-pair tiles within the list cutoff in the last call (synchronises)."""
         return None if self.status is None else int(self.status[1].item())
@@ -125,6 +170,30 @@ class FusedNonbondedPME(nn.Module):
         if st & COLUMN_OVERFLOW:
             raise RuntimeError("FusedNonbondedPME: an x-y column holds more than 8192 atoms (strongly "
                                "inhomogeneous system); results are not reproducible bit for bit")
+
+
+class _FusedEnergyForces(torch.autograd.Function):
+    """(E, F, terms) with a hand-written vector-Jacobian product (see the module docstring)."""
+
+    @staticmethod
+    def forward(ctx, module, coords, charges, sigma, epsilon, box):
+        E, F, terms = module._forward_raw(coords, box, charges, sigma, epsilon)
+        ctx.module = module
+        ctx.save_for_backward(coords, charges, sigma, epsilon, box)
+        terms = terms.clone()
+        ctx.mark_non_differentiable(terms)
+        return E.clone(), F, terms
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, grad_energy, grad_forces, grad_terms):
+        coords, charges, sigma, epsilon, box = ctx.saved_tensors
+        needs = ctx.needs_input_grad[1:5]
+        if not any(needs):
+            return None, None, None, None, None, None
+        d = ctx.module._backward_raw(coords, box, charges, sigma, epsilon, grad_energy, grad_forces, needs)
+        d = [None if x is None else x.to(t.dtype) for x, t in zip(d, (coords, charges, sigma, epsilon))]
+        return (None, *d, None)
 
 
 def for_workload(w, precision="mixed"):

@@ -106,3 +106,19 @@ def make_workload(name, **overrides):
     d["alpha"] = ewald_alpha(cfg["cutoff"], cfg["tol"])
     d["name"] = name
     return d
+
+
+def differentiation_inputs(w, seed=7):
+    """Force-matching inputs for a water workload: per-atom charges perturbed by N(0, 0.02) and
+    kept neutral per molecule, sigma and epsilon scaled by U(0.95, 1.05), and a fixed target
+    force field N(0, 50 kJ/mol/nm).  Returns (charges, sigma, epsilon, target_forces)."""
+    rng = np.random.default_rng(seed)
+    n = w["natoms"]
+    dq = rng.normal(0.0, 0.02, n).reshape(-1, 3)
+    dq -= dq.mean(axis=1, keepdims=True)
+    q = w["charges"] + dq.reshape(-1)
+    sig = w["sigma"] * rng.uniform(0.95, 1.05, n)
+    eps = w["epsilon"] * rng.uniform(0.95, 1.05, n)
+    target = rng.normal(0.0, 50.0, (n, 3))
+    return q, sig, eps, target
+
