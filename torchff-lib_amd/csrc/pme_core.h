@@ -278,12 +278,14 @@ __device__ __forceinline__ T signed_m(int m, int K) {
   return (T)(m <= K / 2 ? m : m - K);
 }
 
+// One tile of a pass (the body of k_fft_pass): tile index and k-space energy slot explicit, so
+// that batched launches over many grids can drive it.
 template <typename T, int MODE>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict__ in_v, void* __restrict__ out_v,
-                                                          PassArgs pa, FftPlan plan, ConvArgs cv,
-                                                          const T* __restrict__ modx, const T* __restrict__ mody,
-                                                          const T* __restrict__ modz, int modz_half,
-                                                          double* __restrict__ e_part) {
+__device__ __forceinline__ void fft_pass_tile(const void* __restrict__ in_v, void* __restrict__ out_v,
+                                              const PassArgs& pa, const FftPlan& plan, const ConvArgs& cv,
+                                              const T* __restrict__ modx, const T* __restrict__ mody,
+                                              const T* __restrict__ modz, int modz_half, int tile,
+                                              double* __restrict__ e_slot) {
   extern __shared__ unsigned char lds_raw[];
   const int K = pa.K, LS = pa.LS;
   const int nl = pa.AB * pa.CB;
@@ -293,7 +295,6 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
   // twiddles exp(-2 pi i m / K)
   fill_twiddles(tw, K);
   const int ntc = (pa.C + pa.CB - 1) / pa.CB;
-  const int tile = blockIdx.x;
   const int a0 = (tile / ntc) * pa.AB;
   const int c0 = (tile % ntc) * pa.CB;
   const int nelem = nl * K;
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
     cx<T>* other = (res == bufA) ? bufB : bufA;
     res = fft_lines<T, 1>(res, other, tw, K, LS, nl_shift, plan);
     double tot = tff_block_sum(e_acc);
-    if (threadIdx.x == 0) e_part[blockIdx.x] = tot;
+    if (threadIdx.x == 0) *e_slot = tot;
   }
   // store
   for (int idx = threadIdx.x; idx < nelem; idx += blockDim.x) {
@@ -380,6 +381,16 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict
         reinterpret_cast<cx<T>*>(out_v)[g] = v;
     }
   }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(const void* __restrict__ in_v, void* __restrict__ out_v,
+                                                          PassArgs pa, FftPlan plan, ConvArgs cv,
+                                                          const T* __restrict__ modx, const T* __restrict__ mody,
+                                                          const T* __restrict__ modz, int modz_half,
+                                                          double* __restrict__ e_part) {
+  fft_pass_tile<T, MODE>(in_v, out_v, pa, plan, cv, modx, mody, modz, modz_half, blockIdx.x,
+                         MODE == PASS_X_CONV ? e_part + blockIdx.x : nullptr);
 }
 
 // Real-to-complex transform along the contiguous (z) axis of a real grid of nlines lines of

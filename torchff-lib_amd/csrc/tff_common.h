@@ -23,6 +23,7 @@ enum TffStatus {
   TFF_ERR_WORKSPACE = 2,
   TFF_ERR_LAUNCH = 3,
   TFF_ERR_UNSUPPORTED = 4,
+  TFF_FULL = 5,  // a capacity would be exceeded (nothing was changed)
 };
 
 struct TffBox {

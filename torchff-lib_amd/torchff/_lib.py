@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libtorchff_hip.so")
 
 F32, F64 = 0, 1
 NBLIST_NSQUARED, NBLIST_CELL_LIST = 0, 1
+SERVE_FULL = 5  # tff_serve_table_add: the system does not fit the capacities
 PAIR_NONBONDED, PAIR_EWALD_LJ, PAIR_COUL_EWALD, PAIR_LJ_TYPES, PAIR_LJ_PAIRS = 0, 1, 2, 3, 4
 
 _vp = ctypes.c_void_p
@@ -44,6 +45,14 @@ _SIGNATURES = {
     "tff_fused_energy_forces_backward": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i,
                                               _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz,
                                               _vp]),
+    "tff_serve_table_bytes": (_sz, [_i]),
+    "tff_serve_table_reset": (None, [_vp]),
+    "tff_serve_table_totals": (_i, [_vp, _vp]),
+    "tff_serve_table_add": (_i, [_vp, _i, _i, _i, _i, _i, _ll, _i, _i, _vp, _vp, _vp, _d, _d]),
+    "tff_serve_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _ll]),
+    "tff_serve_upload": (_i, [_vp, _i, _i, _i, _i, _i, _ll, _vp, _sz, _vp]),
+    "tff_serve_energy_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _i, _i, _i, _i, _ll, _vp, _vp,
+                                     _vp, _sz, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,356 @@
+"""Batched serving of energy + force evaluations for many small periodic water systems.
+
+The reference evaluates one system per call of its TIP3P composition
+(examples/tip3p/model.py:100-123).  Here requests for systems of different
+sizes arrive over time and are evaluated in batches by one fixed sequence of
+launches (``tff_serve_energy_forces``), which is captured once per batch slot
+into a HIP graph and replayed for every batch:
+
+* a **slot** owns a stream, a device workspace sized by fixed capacities
+  (systems, 64-atom tasks, FFT tiles per pass, largest grid edge, grid points,
+  atoms), pinned host staging for inputs / outputs, the host table of the
+  batch, and the captured graph;
+* **admission by HBM footprint**: a request joins the batch being formed only
+  if its atoms, grid points, tasks and tiles fit what is left of the slot's
+  capacities (``tff_serve_table_add`` refuses it otherwise); the capacities
+  are the slot's device memory, fixed when the engine is built;
+* **continuous batching**: while batches run on the device, requests that
+  arrive wait in a queue; whenever a slot is free, a new batch is formed from
+  the queue in arrival order and launched on that slot's stream, so several
+  batches overlap on the device;
+* each system's result depends only on its own atoms (fixed-order sums and
+  integer fixed-point grid accumulation), so it is the same alone and in any
+  mix.
+
+Host-side PME grids follow the serving rule: spacing <= ``spacing`` nm rounded up
+to 2^a 3^b 5^c.
+"""
+
+import ctypes
+import math
+import time
+from collections import deque
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import bsplines
+from .workloads import COULOMB_KJMOL, box_length_for_density, ewald_alpha, water_box
+
+
+def smooth_size(n):
+    """True when n = 2^a 3^b 5^c."""
+    for f in (2, 3, 5):
+        while n % f == 0 and n > 1:
+            n //= f
+    return n == 1
+
+
+def pme_grid_size(length, spacing=0.1):
+    """Smallest K >= length / spacing of the form 2^a 3^b 5^c."""
+    k = max(1, int(math.ceil(length / spacing - 1e-9)))
+    while not smooth_size(k):
+        k += 1
+    return k
+
+
+class ModuliTable:
+    """|theta(m)|, m = 0..K-1, of every 2^a 3^b 5^c grid size K <= max_k, concatenated on the device."""
+
+    def __init__(self, order, max_k, dtype, device):
+        parts, self.offsets, off = [], {}, 0
+        for k in range(1, max_k + 1):
+            if smooth_size(k):
+                self.offsets[k] = off
+                parts.append(np.sqrt(bsplines.theta_sq(k, order)))
+                off += k
+        self.table = torch.tensor(np.concatenate(parts), dtype=dtype, device=device)
+
+
+class Request:
+    """One energy + forces evaluation: a periodic orthorhombic box of water-like molecules."""
+
+    __slots__ = ("rid", "arrival", "coords", "charges", "sigma", "epsilon", "molecule", "box_lengths",
+                 "grid", "natoms", "t_submit", "t_done", "energies", "forces", "_L", "_K", "_mod")
+
+    def __init__(self, rid, coords, charges, sigma, epsilon, molecule, box_lengths, grid, arrival=0.0):
+        self.rid = rid
+        self.arrival = float(arrival)
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.charges = np.ascontiguousarray(charges, dtype=np.float64)
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        self.epsilon = np.ascontiguousarray(epsilon, dtype=np.float64)
+        self.molecule = np.ascontiguousarray(molecule, dtype=np.int32)
+        self.box_lengths = tuple(float(x) for x in box_lengths)
+        self.grid = tuple(int(k) for k in grid)
+        self.natoms = len(self.charges)
+        self.t_submit = self.t_done = None
+        self.energies = self.forces = None
+        self._L = (ctypes.c_double * 3)(*self.box_lengths)
+        self._K = (ctypes.c_int * 3)(*self.grid)
+        self._mod = None
+
+
+def water_request(rid, n_waters, seed, spacing=0.1, arrival=0.0):
+    """A request for a box of n_waters waters at 1 g/cm^3 (the seeded construction of
+    torchff.workloads.water_box), with its grid from the spacing rule."""
+    Lb = box_length_for_density(n_waters)
+    w = water_box(n_waters, Lb, seed=seed)
+    K = pme_grid_size(Lb, spacing)
+    mol = np.arange(w["natoms"], dtype=np.int32) // 3
+    return Request(rid, w["coords"], w["charges"], w["sigma"], w["epsilon"], mol, (Lb,) * 3, (K,) * 3, arrival)
+
+
+def request_stream(n_requests=4096, rate=2000.0, min_waters=20, max_waters=1000, seed=11, spacing=0.1):
+    """Poisson arrivals at ``rate`` per second; sizes log-uniform in [min_waters, max_waters]."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    sizes = np.exp(rng.uniform(math.log(min_waters), math.log(max_waters), n_requests))
+    sizes = np.clip(np.rint(sizes).astype(int), min_waters, max_waters)
+    return [water_request(i, int(n), seed * 100003 + i, spacing, arrivals[i]) for i, n in enumerate(sizes)]
+
+
+class _Slot:
+    def __init__(self, eng, index):
+        self.eng = eng
+        self.index = index
+        dev = eng.device
+        c = eng.caps
+        self.stream = torch.cuda.Stream(device=dev)
+        lib = L.load()
+        self.table = torch.empty(lib.tff_serve_table_bytes(c["max_systems"]), dtype=torch.uint8, pin_memory=True)
+        self.table_ptr = ctypes.c_void_p(self.table.data_ptr())
+        nbytes = lib.tff_serve_workspace_bytes(eng.cd, *eng.cap_args)
+        if nbytes == 0:
+            raise RuntimeError("ServingEngine: capacities rejected (max_k too large for one FFT tile's LDS)")
+        self.ws = L.workspace(nbytes, dev)
+        na = c["max_atoms"]
+        f64 = dict(dtype=torch.float64)
+        self.h_in = torch.empty(6 * na, pin_memory=True, **f64)  # coords (3 na), q, sigma, epsilon
+        self.h_mol = torch.empty(na, dtype=torch.int32, pin_memory=True)
+        self.h_out = torch.empty(3 * na, pin_memory=True, **f64)
+        self.h_energy = torch.empty(5 * c["max_systems"], pin_memory=True, **f64)
+        self.d_in = torch.empty(6 * na, device=dev, **f64)
+        self.d_mol = torch.empty(na, dtype=torch.int32, device=dev)
+        self.d_out = torch.empty(3 * na, device=dev, **f64)
+        self.d_energy = torch.empty(5 * c["max_systems"], device=dev, **f64)
+        self.n_in = self.h_in.numpy()
+        self.n_mol = self.h_mol.numpy()
+        self.n_out = self.h_out.numpy()
+        self.n_energy = self.h_energy.numpy()
+        self.d_x = self.d_in[: 3 * na]
+        self.d_q = self.d_in[3 * na: 4 * na]
+        self.d_s = self.d_in[4 * na: 5 * na]
+        self.d_e = self.d_in[5 * na:]
+        self.requests = []
+        self.natoms = 0
+        self.event = torch.cuda.Event()
+        self.graph = None
+        self.bytes = (nbytes + self.d_in.numel() * 8 + self.d_mol.numel() * 4 + self.d_out.numel() * 8
+                      + self.d_energy.numel() * 8)
+
+    def launch_call(self):
+        eng = self.eng
+        rc = L.load().tff_serve_energy_forces(
+            L.F64, eng.cd, L.ptr(self.d_x), L.ptr(self.d_q), L.ptr(self.d_s), L.ptr(self.d_e), L.ptr(self.d_mol),
+            eng.coul, eng.order, L.ptr(eng.moduli.table), *eng.cap_args, L.ptr(self.d_energy), L.ptr(self.d_out),
+            L.ptr(self.ws), self.ws.numel(), ctypes.c_void_p(self.stream.cuda_stream))
+        L.check(rc, "serve energy+forces")
+
+    def reset(self):
+        L.load().tff_serve_table_reset(self.table_ptr)
+        self.requests = []
+        self.natoms = 0
+
+    def try_add(self, r):
+        """Adds a request to the batch being formed; False when it does not fit."""
+        eng = self.eng
+        if self.natoms + r.natoms > eng.caps["max_atoms"]:
+            return False
+        if r._mod is None:
+            if any(k not in eng.moduli.offsets for k in r.grid):
+                return False
+            r._mod = (ctypes.c_int * 3)(*(eng.moduli.offsets[k] for k in r.grid))
+        rc = L.load().tff_serve_table_add(self.table_ptr, eng.cd, *eng.cap_args, self.natoms, r.natoms, r._L, r._K,
+                                          r._mod, eng.cutoff, eng.alpha)
+        if rc == L.SERVE_FULL:
+            return False
+        L.check(rc, "serve table")
+        na, a0, n = eng.caps["max_atoms"], self.natoms, r.natoms
+        self.n_in[3 * a0: 3 * (a0 + n)] = r.coords.reshape(-1)
+        self.n_in[3 * na + a0: 3 * na + a0 + n] = r.charges
+        self.n_in[4 * na + a0: 4 * na + a0 + n] = r.sigma
+        self.n_in[5 * na + a0: 5 * na + a0 + n] = r.epsilon
+        self.n_mol[a0: a0 + n] = r.molecule
+        self.requests.append(r)
+        self.natoms += n
+        return True
+
+    def launch(self):
+        """Uploads the batch, runs it (graph replay when captured) and queues the copies back."""
+        eng = self.eng
+        na, n, ns = eng.caps["max_atoms"], self.natoms, len(self.requests)
+        lib = L.load()
+        with torch.cuda.stream(self.stream):
+            self.d_in[: 3 * n].copy_(self.h_in[: 3 * n], non_blocking=True)
+            for k in (3, 4, 5):
+                self.d_in[k * na: k * na + n].copy_(self.h_in[k * na: k * na + n], non_blocking=True)
+            self.d_mol[:n].copy_(self.h_mol[:n], non_blocking=True)
+            rc = lib.tff_serve_upload(self.table_ptr, eng.cd, *eng.cap_args, L.ptr(self.ws), self.ws.numel(),
+                                      ctypes.c_void_p(self.stream.cuda_stream))
+            L.check(rc, "serve upload")
+            if self.graph is not None:
+                self.graph.replay()
+            else:
+                self.launch_call()
+            self.h_out[: 3 * n].copy_(self.d_out[: 3 * n], non_blocking=True)
+            self.h_energy[: 5 * ns].copy_(self.d_energy[: 5 * ns], non_blocking=True)
+            self.event.record(self.stream)
+
+    def finish(self, now):
+        a0 = 0
+        for s, r in enumerate(self.requests):
+            r.energies = self.n_energy[5 * s: 5 * s + 5].copy()
+            r.forces = self.n_out[3 * a0: 3 * (a0 + r.natoms)].reshape(-1, 3).copy()
+            r.t_done = now
+            a0 += r.natoms
+        done = self.requests
+        self.reset()
+        return done
+
+
+class ServingEngine:
+    """Continuous-batching evaluator (see the module docstring).
+
+    capacities per slot: max_systems, max_atoms, max_grid_points, max_k (largest grid edge);
+    the task and tile capacities follow from them.  precision "mixed" (float32 arithmetic,
+    int64 fixed-point grid, float64 sums) or "fp64".  Results: energies per request
+    [real space, 0, reciprocal, self, total] (kJ/mol) and forces (kJ/mol/nm).
+    """
+
+    def __init__(self, slots=2, max_systems=64, max_atoms=65536, max_grid_points=1 << 21, max_k=64,
+                 cutoff=0.9, ewald_alpha_value=None, order=5, precision="mixed", coul=COULOMB_KJMOL,
+                 use_graph=True, device="cuda"):
+        if precision not in ("mixed", "fp64"):
+            raise ValueError("precision must be 'mixed' or 'fp64'")
+        L.load()
+        self.device = torch.device(device)
+        self.cd = L.F32 if precision == "mixed" else L.F64
+        self.precision = precision
+        self.cutoff = float(cutoff)
+        self.alpha = float(ewald_alpha(cutoff) if ewald_alpha_value is None else ewald_alpha_value)
+        self.order = int(order)
+        self.coul = float(coul)
+        max_tasks = max_atoms // 64 + max_systems
+        # an FFT pass of a K^3 grid (K >= 9) has at most K^2 / 16 + 2 K tiles of 16 lines
+        max_tiles = max_grid_points // 144 + 2 * max_systems * max_k
+        self.caps = dict(max_systems=max_systems, max_tasks=max_tasks, max_tiles=max_tiles, max_k=max_k,
+                         max_grid_points=max_grid_points, max_atoms=max_atoms)
+        self.cap_args = (max_systems, max_tasks, max_tiles, max_k, max_grid_points)
+        cdt = torch.float32 if precision == "mixed" else torch.float64
+        self.moduli = ModuliTable(self.order, max_k, cdt, self.device)
+        self.slots = [_Slot(self, i) for i in range(slots)]
+        self.use_graph = use_graph
+        if use_graph:
+            self._capture()
+
+    @property
+    def hbm_bytes(self):
+        return sum(s.bytes for s in self.slots) + self.moduli.table.numel() * self.moduli.table.element_size()
+
+    def _capture(self):
+        for s in self.slots:
+            s.reset()
+            # an empty table: the captured launches read the batch from the workspace at replay
+            with torch.cuda.stream(s.stream):
+                L.check(L.load().tff_serve_upload(s.table_ptr, self.cd, *self.cap_args, L.ptr(s.ws), s.ws.numel(),
+                                                  ctypes.c_void_p(s.stream.cuda_stream)), "serve upload")
+                s.launch_call()
+            s.stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=s.stream):
+                s.launch_call()
+            s.graph = g
+        torch.cuda.synchronize(self.device)
+
+    def fits_alone(self, r):
+        """Whether the request fits an empty slot (else it can never be served)."""
+        s = self.slots[0]
+        if s.requests:
+            raise RuntimeError("fits_alone needs an idle engine")
+        ok = s.try_add(r)
+        s.reset()
+        return ok
+
+    def evaluate(self, requests):
+        """Evaluates the requests now (batched over the slots), blocking; returns them."""
+        queue = deque(requests)
+        busy = []
+        free = list(self.slots)
+        while queue or busy:
+            while queue and free:
+                s = free.pop()
+                while queue and s.try_add(queue[0]):
+                    queue.popleft()
+                if not s.requests:
+                    raise RuntimeError(f"request {queue[0].rid} ({queue[0].natoms} atoms, grid {queue[0].grid}) "
+                                       "exceeds the slot capacities")
+                s.launch()
+                busy.append(s)
+            s = busy.pop(0)
+            s.event.synchronize()
+            s.finish(time.perf_counter())
+            free.append(s)
+        return requests
+
+    def serve(self, requests, realtime=True, max_batch_systems=None):
+        """Runs the request stream: request i becomes visible at t0 + arrival_i (realtime) or at
+        once; batches are formed whenever a slot is free.  Returns the requests with t_submit /
+        t_done set (seconds from t0) and the number of batches launched."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        limit = max_batch_systems or self.caps["max_systems"]
+        for r in order:
+            if not self.fits_alone(r):
+                raise RuntimeError(f"request {r.rid} exceeds the slot capacities")
+        pending = deque()
+        busy = deque()
+        free = deque(self.slots)
+        nxt, done, nb = 0, 0, 0
+        t0 = time.perf_counter()
+        while done < len(order):
+            now = time.perf_counter() - t0
+            while nxt < len(order) and (not realtime or order[nxt].arrival <= now):
+                order[nxt].t_submit = order[nxt].arrival if realtime else 0.0
+                pending.append(order[nxt])
+                nxt += 1
+            progressed = False
+            for s in [b for b in busy if b.event.query()]:
+                busy.remove(s)
+                done += len(s.finish(time.perf_counter() - t0))
+                free.append(s)
+                progressed = True
+            while pending and free:
+                s = free.popleft()
+                while pending and len(s.requests) < limit and s.try_add(pending[0]):
+                    pending.popleft()
+                s.launch()
+                busy.append(s)
+                nb += 1
+                progressed = True
+            if not progressed:
+                if busy:
+                    time.sleep(20e-6)
+                elif nxt < len(order) and realtime:
+                    wait = order[nxt].arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 1e-3))
+        return order, nb
+
+
+def latency_stats(reqs):
+    lat = np.array([r.t_done - r.t_submit for r in reqs])
+    span = max(r.t_done for r in reqs) - min(r.t_submit for r in reqs)
+    return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                mean_ms=float(lat.mean() * 1e3), evals_per_s=float(len(reqs) / span), span_s=float(span))
