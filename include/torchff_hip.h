@@ -177,47 +177,50 @@ int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void
  * cutoff, and the pairs of one molecule are excluded on their nearest image (the correction
  * k q_i q_j (erfc(alpha r) - 1) / r, examples/tip3p/model.py:110-112 and
  * torchff/multipoles.py:256-257; no LJ there).  The reference evaluates one system per operator
- * call (examples/tip3p/model.py:100-123); this entry point has no single counterpart there: it
- * serves that per-system evaluation to many systems at once.
+ * call (examples/tip3p/model.py:100-123); these entry points have no single counterpart there:
+ * they serve that per-system evaluation to many systems at once.
  *
- * A batch is described by a host table (head + one record per system) that the caller fills
- * with tff_serve_table_reset / tff_serve_table_add, copies into the workspace with
- * tff_serve_upload, and evaluates with tff_serve_energy_forces.  Grids of all launches are
- * sized by the capacities (max_systems, max_tasks = 64-atom blocks, max_tiles = FFT tiles per
- * pass, max_k = largest grid edge, max_grid_points), never by the batch, so one upload +
- * energy_forces sequence per batch can replay a captured HIP graph of the energy_forces call.
- * The same five capacities must be passed to every call on one workspace. */
+ * Capacities (the same four on every call for one workspace): max_systems, max_atoms (atom
+ * records), max_k (largest grid edge; float32: <= 62, float64: <= 32, one x-slab transform in
+ * 64 KiB of LDS), max_grid_points (summed K0 K1 K2).  Launch grids follow them, never
+ * the batch, so the tff_serve_energy_forces call can be captured once into a HIP graph and
+ * replayed after each tff_serve_upload.
+ *
+ * A batch is a host table, tff_serve_table_bytes long (pinned memory for asynchronous uploads):
+ * a head, one record per system (tff_serve_table_add), and from byte tff_serve_atoms_offset on
+ * the atom records, 8 float64 each: x, y, z (nm), charge (e), sigma (nm), epsilon (kJ/mol),
+ * molecule id (equal within a molecule: the exclusion group), 0.  tff_serve_upload copies the
+ * head, the records and the atom records up to the highest one used into the workspace, with one
+ * copy; tff_serve_energy_forces evaluates it into out (device float64): 5 values per system in
+ * table order [real space (LJ + Coulomb incl. the exclusion correction), 0, reciprocal, self,
+ * total] at out[5 s], then from out[5 max_systems] the forces -dE/dr (3 per atom record, rows of
+ * records in no system not written).  A new workspace is zeroed once by
+ * tff_serve_workspace_init; the charge grid is cleaned by each evaluation for the next. */
 #define TFF_FULL 5 /* tff_serve_table_add: the system does not fit the capacities (table unchanged) */
 
-/* bytes of a host table for max_systems systems (pinned host memory for asynchronous uploads) */
-size_t tff_serve_table_bytes(int max_systems);
+size_t tff_serve_table_bytes(int max_systems, int max_atoms);
+size_t tff_serve_atoms_offset(int max_systems);
 void tff_serve_table_reset(void* table_host);
-/* totals_host (7 int64): systems, atoms, 64-atom tasks, FFT tiles of the z, y and x passes,
- * grid points of the table */
+/* totals_host (8 int64): systems, atoms, 64-atom tasks, FFT tiles of the z, y and x passes, grid
+ * points, one past the highest atom record used */
 int tff_serve_table_totals(const void* table_host, long long* totals_host);
-/* Appends a system whose atoms are rows atom_off .. atom_off + natoms - 1 of the arrays passed to
- * tff_serve_energy_forces; box_lengths_host (3 doubles, nm), K_host (3 grid sizes 2^a 3^b 5^c,
- * each <= max_k), moduli_off_host (3 offsets into the moduli table where the |theta(m)|,
- * m = 0..K-1, of each axis start, as tff_pme_energy's xmoduli).  Returns 0, TFF_FULL, or an
- * error. */
-int tff_serve_table_add(void* table_host, int compute_dtype, int max_systems, int max_tasks, int max_tiles,
-                        int max_k, long long max_grid_points, int atom_off, int natoms,
-                        const double* box_lengths_host, const int* K_host, const int* moduli_off_host,
-                        double cutoff, double ewald_alpha);
-/* 0 when max_k needs more than 64 KiB of LDS per FFT tile (float32: max_k <= 247, float64: 123) */
-size_t tff_serve_workspace_bytes(int compute_dtype, int max_systems, int max_tasks, int max_tiles, int max_k,
+/* Appends a system of atom records atom_off .. atom_off + natoms - 1; box_lengths_host (3
+ * doubles, nm), K_host (3 grid sizes 2^a 3^b 5^c, each <= max_k), moduli_off_host (3 offsets
+ * into the moduli table where |theta(m)|, m = 0..K-1, of each axis start, as tff_pme_energy's
+ * xmoduli, in compute_dtype).  Returns 0, TFF_FULL, or an error. */
+int tff_serve_table_add(void* table_host, int compute_dtype, int max_systems, int max_atoms, int max_k,
+                        long long max_grid_points, int atom_off, int natoms, const double* box_lengths_host,
+                        const int* K_host, const int* moduli_off_host, double cutoff, double ewald_alpha);
+/* 0 when the capacities are rejected */
+size_t tff_serve_workspace_bytes(int compute_dtype, int max_systems, int max_atoms, int max_k,
                                  long long max_grid_points);
-int tff_serve_upload(const void* table_host, int compute_dtype, int max_systems, int max_tasks, int max_tiles,
-                     int max_k, long long max_grid_points, void* workspace, size_t workspace_bytes,
-                     tff_stream_t stream);
-/* molecule: int32 per atom, equal within a molecule (the exclusion groups).  energies_out
- * (device, 5 doubles per table record, in table order): real space (LJ + Coulomb incl. the
- * exclusion correction), 0, reciprocal, self, total.  forces_out (io_dtype, rows as coords) =
- * -dE/dr; rows of atoms in no record are not written.  Spline order 3..8. */
-int tff_serve_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
-                            const void* sigma, const void* epsilon, const int32_t* molecule, double coul_constant,
-                            int order, const void* moduli, int max_systems, int max_tasks, int max_tiles, int max_k,
-                            long long max_grid_points, double* energies_out, void* forces_out, void* workspace,
+int tff_serve_workspace_init(int compute_dtype, int max_systems, int max_atoms, int max_k, long long max_grid_points,
+                             void* workspace, size_t workspace_bytes, tff_stream_t stream);
+int tff_serve_upload(const void* table_host, int compute_dtype, int max_systems, int max_atoms, int max_k,
+                     long long max_grid_points, void* workspace, size_t workspace_bytes, tff_stream_t stream);
+/* Spline order 3..8.  Calls that overlap on one workspace race. */
+int tff_serve_energy_forces(int compute_dtype, double coul_constant, int order, const void* moduli, int max_systems,
+                            int max_atoms, int max_k, long long max_grid_points, double* out, void* workspace,
                             size_t workspace_bytes, tff_stream_t stream);
 
 #ifdef __cplusplus

@@ -20,7 +20,8 @@ RC = 0.9
 
 
 def _oracle(r, alpha, order=5):
-    w = dict(coords=r.coords, box=np.diag(r.box_lengths), charges=r.charges, sigma=r.sigma, epsilon=r.epsilon)
+    rec = r.records
+    w = dict(coords=rec[:, :3], box=np.diag(r.box_lengths), charges=rec[:, 3], sigma=rec[:, 4], epsilon=rec[:, 5])
     return images.energy_forces(w, RC, alpha, r.grid, order)
 
 
@@ -87,7 +88,7 @@ def test_graph_replays_match_direct_launches():
 
 def test_serve_stream_realtime():
     reqs = request_stream(200, rate=2000.0, min_waters=20, max_waters=1000, seed=7)
-    eng = ServingEngine(slots=2, max_k=64)
+    eng = ServingEngine(slots=2, max_k=32)
     done, nb = eng.serve(reqs, realtime=True)
     assert all(r.t_done is not None and r.t_done >= r.t_submit for r in done)
     st = latency_stats(done)

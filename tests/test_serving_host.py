@@ -15,13 +15,13 @@ def lib():
     return L.load(require_gpu=False)
 
 
-def _table(lib, max_sys):
-    buf = np.zeros(lib.tff_serve_table_bytes(max_sys), dtype=np.uint8)
+def _table(lib, max_sys, max_atoms=1 << 14):
+    buf = np.zeros(lib.tff_serve_table_bytes(max_sys, max_atoms), dtype=np.uint8)
     return buf, ctypes.c_void_p(buf.ctypes.data)
 
 
 def _totals(lib, tp):
-    t = np.zeros(7, dtype=np.int64)
+    t = np.zeros(8, dtype=np.int64)
     assert lib.tff_serve_table_totals(tp, ctypes.c_void_p(t.ctypes.data)) == 0
     return t
 
@@ -49,7 +49,7 @@ def test_grid_sizes():
 
 def test_table_totals_and_offsets(lib):
     buf, tp = _table(lib, 8)
-    caps = (8, 1000, 10000, 64, 1 << 20)
+    caps = (8, 1 << 14, 32, 1 << 20)
     lib.tff_serve_table_reset(tp)
     sizes = [(60, 0.843, 9), (3000, 3.1, 32), (300, 1.44, 15)]
     off = 0
@@ -61,17 +61,21 @@ def test_table_totals_and_offsets(lib):
     tiles = np.sum([_tiles(K) for _, _, K in sizes], axis=0)
     assert list(t[3:6]) == list(tiles)
     assert t[6] == 9 ** 3 + 32 ** 3 + 15 ** 3
+    assert t[7] == 3360
     lib.tff_serve_table_reset(tp)
-    assert list(_totals(lib, tp)) == [0] * 7
+    assert list(_totals(lib, tp)) == [0] * 8
+    # the atom records start after the head and the records, 256-byte aligned
+    off = lib.tff_serve_atoms_offset(8)
+    assert off % 256 == 0 and lib.tff_serve_table_bytes(8, 1 << 14) == off + 64 * (1 << 14)
 
 
 def test_table_refuses_what_does_not_fit(lib):
-    buf, tp = _table(lib, 2)
-    caps = (2, 60, 10000, 64, 40000)
+    buf, tp = _table(lib, 2, 4000)
+    caps = (2, 4000, 32, 40000)
     lib.tff_serve_table_reset(tp)
-    assert _add(lib, tp, caps, 0, 3000, 3.1, 32) == 0          # 47 tasks, 32768 points
+    assert _add(lib, tp, caps, 0, 3000, 3.1, 32) == 0          # 3000 atoms, 32768 points
     before = _totals(lib, tp).copy()
-    assert _add(lib, tp, caps, 3000, 3000, 3.1, 32) == L.SERVE_FULL  # tasks and grid exceeded
+    assert _add(lib, tp, caps, 3000, 1200, 1.6, 16) == L.SERVE_FULL  # atoms exceeded
     assert _add(lib, tp, caps, 3000, 900, 2.08, 24) == L.SERVE_FULL  # grid points exceeded
     assert list(_totals(lib, tp)) == list(before)                      # unchanged
     assert _add(lib, tp, caps, 3000, 60, 0.843, 9) == 0
@@ -80,19 +84,22 @@ def test_table_refuses_what_does_not_fit(lib):
 
 def test_table_rejects_bad_systems(lib):
     buf, tp = _table(lib, 4)
-    caps = (4, 100, 1000, 32, 1 << 16)
+    caps = (4, 1 << 14, 32, 1 << 16)
     lib.tff_serve_table_reset(tp)
     assert _add(lib, tp, caps, 0, 60, 0.843, 14) not in (0, L.SERVE_FULL)   # 14 = 2 x 7
     assert _add(lib, tp, caps, 0, 60, 0.843, 36) not in (0, L.SERVE_FULL)   # above max_k
     assert _add(lib, tp, caps, 0, 0, 0.843, 9) not in (0, L.SERVE_FULL)     # no atoms
+    assert _add(lib, tp, caps, 0, 60, 0.05, 9) not in (0, L.SERVE_FULL)     # > 16 images per axis
+    assert list(_totals(lib, tp)) == [0] * 8
 
 
 def test_workspace_lds_limit(lib):
-    # one FFT tile of 16 lines must fit 64 KiB of LDS: max_k <= 247 (float32), 123 (float64)
-    assert lib.tff_serve_workspace_bytes(L.F32, 4, 100, 100, 247, 1 << 20) > 0
-    assert lib.tff_serve_workspace_bytes(L.F32, 4, 100, 100, 248, 1 << 20) == 0
-    assert lib.tff_serve_workspace_bytes(L.F64, 4, 100, 100, 123, 1 << 20) > 0
-    assert lib.tff_serve_workspace_bytes(L.F64, 4, 100, 100, 124, 1 << 20) == 0
+    # a slab transform (two power-of-two line sets of edge + 1 values) must fit 64 KiB of LDS:
+    # max_k <= 62 (float32), 32 (float64)
+    assert lib.tff_serve_workspace_bytes(L.F32, 4, 1000, 62, 1 << 20) > 0
+    assert lib.tff_serve_workspace_bytes(L.F32, 4, 1000, 63, 1 << 20) == 0
+    assert lib.tff_serve_workspace_bytes(L.F64, 4, 1000, 32, 1 << 20) > 0
+    assert lib.tff_serve_workspace_bytes(L.F64, 4, 1000, 33, 1 << 20) == 0
 
 
 def test_moduli_table_offsets():

@@ -25,6 +25,7 @@
 
 #include <mutex>
 
+#include "pair_math.h"
 #include "pme_core.h"
 #include "tff_common.h"
 #include "torchff_hip.h"
@@ -590,44 +591,6 @@ __global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ nc
 }
 
 // ---------------------------------------------------------------- 4. tile kernel
-// erfc(u) and exp(-u^2) for u >= 0.
-// float: exp(-u^2) = 2^(-u^2 log2 e) with the product split into a rounded part and its fma
-// remainder (plus the rounding of log2 e in float), the rounded part through the hardware
-// base-2 exponential and the remainder as a first-order factor; erfc(u) exp(u^2) =
-// sum_k c_k t^k, t = 1 / (1 + 0.39 u) (hardware reciprocal), a degree-8 polynomial fitted
-// to the scaled complementary error function on [0, 3.6] by iteratively reweighted least
-// squares (relative error 1.3e-9 in exact arithmetic, about 1.5e-6 evaluated in float32,
-// against scipy.special.erfc).  double: the correctly rounded library functions.
-__device__ __forceinline__ void erfc_exp(float u, float& ec, float& em) {
-  const float v = u * u;
-  const float hi = v * -1.44269502f;
-  float lo = fmaf(v, -1.44269502f, -hi);
-  lo = fmaf(v, -1.92596299e-8f, lo);
-  const float e = __builtin_amdgcn_exp2f(hi);
-  em = fmaf(e, lo * 0.693147181f, e);
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.39f, u, 1.0f));
-  float p = 9.27117256e-02f;
-  p = fmaf(p, t, -5.29840954e-01f);
-  p = fmaf(p, t, 1.06552963e+00f);
-  p = fmaf(p, t, -8.61721388e-01f);
-  p = fmaf(p, t, 7.81784359e-01f);
-  p = fmaf(p, t, -4.21555312e-02f);
-  p = fmaf(p, t, 2.82117988e-01f);
-  p = fmaf(p, t, 2.10995424e-01f);
-  p = fmaf(p, t, 5.78745155e-04f);
-  ec = p * em;
-}
-__device__ __forceinline__ void erfc_exp(double u, double& ec, double& em) {
-  em = exp(-u * u);
-  ec = erfc(u);
-}
-// hardware reciprocal square root refined by one Newton step (Lennard-Jones raises its error
-// to the sixth power)
-__device__ __forceinline__ float rsqrt_t(float x) {
-  const float y = __builtin_amdgcn_rsqf(x);
-  return y * fmaf(-0.5f * x, y * y, 1.5f);
-}
-__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
 
 // sum over the 8 lanes sharing lane >> 3 (the result lands in all 8 lanes)
 template <int CTRL>
@@ -1713,7 +1676,7 @@ template <typename TC>
 void run_recip_fft(const FusedWs& w, const int K[3], const TffBox& box, double alpha, const FftPlan* pl,
                    const TC* mx, const TC* my, const TC* mz, const unsigned long long* grid_in,
                    double grid_inv_scale, TC* phi_out, double* kparts, const char* prof_name, hipStream_t st) {
-  ConvArgs cv;
+  ConvArgs cv{};
   for (int a = 0; a < 3; ++a) cv.K[a] = K[a];
   for (int k = 0; k < 9; ++k) cv.inv[k] = box.inv[k];
   cv.vol = box.det;

@@ -250,7 +250,7 @@ int pme_energy(const T* coords, const T* q, long long n, const TffBox& box, cons
   int rc = launch_spread<T>(order, coords, q, n, g, w.grid, st);
   if (rc) return rc;
   const size_t ce = sizeof(cx<T>);
-  ConvArgs cv;
+  ConvArgs cv{};
   for (int a = 0; a < 3; ++a) cv.K[a] = K[a];
   for (int k = 0; k < 9; ++k) cv.inv[k] = box.inv[k];
   cv.vol = box.det;

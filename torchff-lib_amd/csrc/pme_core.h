@@ -245,6 +245,7 @@ struct ConvArgs {
   double grid_inv_scale;  // fixed-point scale of that grid (value = integer * grid_inv_scale)
   int nz;                 // z extent of the spectrum: K[2] (full) or K[2] / 2 + 1 (real-input half)
   int half;               // 1: half spectrum along z (energy weight 2 on kz in (0, K[2] / 2))
+  int clear_in;           // forward z pass: zero each fixed-point grid value after reading it
 };
 
 // cos(pi x), sin(pi x) in the transform's precision
@@ -316,9 +317,11 @@ __device__ __forceinline__ void fft_pass_tile(const void* __restrict__ in_v, voi
     if (a < pa.A && c < pa.C) {
       long long g = ((long long)a * K + k) * pa.C + c;
       if (MODE == PASS_Z_FWD) {
-        if (cv.grid_fixed)
-          v.x = (T)((double)reinterpret_cast<const long long*>(in_v)[g] * cv.grid_inv_scale);
-        else
+        if (cv.grid_fixed) {
+          long long* gi = reinterpret_cast<long long*>(const_cast<void*>(in_v));
+          v.x = (T)((double)gi[g] * cv.grid_inv_scale);
+          if (cv.clear_in) gi[g] = 0;  // the value is read by this thread only
+        } else
           v.x = (T)reinterpret_cast<const double*>(in_v)[g];
       } else {
         v = reinterpret_cast<const cx<T>*>(in_v)[g];

@@ -45,14 +45,15 @@ _SIGNATURES = {
     "tff_fused_energy_forces_backward": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i,
                                               _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz,
                                               _vp]),
-    "tff_serve_table_bytes": (_sz, [_i]),
+    "tff_serve_table_bytes": (_sz, [_i, _i]),
+    "tff_serve_atoms_offset": (_sz, [_i]),
     "tff_serve_table_reset": (None, [_vp]),
     "tff_serve_table_totals": (_i, [_vp, _vp]),
-    "tff_serve_table_add": (_i, [_vp, _i, _i, _i, _i, _i, _ll, _i, _i, _vp, _vp, _vp, _d, _d]),
-    "tff_serve_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _ll]),
-    "tff_serve_upload": (_i, [_vp, _i, _i, _i, _i, _i, _ll, _vp, _sz, _vp]),
-    "tff_serve_energy_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _i, _i, _i, _i, _ll, _vp, _vp,
-                                     _vp, _sz, _vp]),
+    "tff_serve_table_add": (_i, [_vp, _i, _i, _i, _i, _ll, _i, _i, _vp, _vp, _vp, _d, _d]),
+    "tff_serve_workspace_bytes": (_sz, [_i, _i, _i, _i, _ll]),
+    "tff_serve_workspace_init": (_i, [_i, _i, _i, _i, _ll, _vp, _sz, _vp]),
+    "tff_serve_upload": (_i, [_vp, _i, _i, _i, _i, _ll, _vp, _sz, _vp]),
+    "tff_serve_energy_forces": (_i, [_i, _d, _i, _vp, _i, _i, _i, _ll, _vp, _vp, _sz, _vp]),
 }
 
 _lib = None

@@ -71,20 +71,24 @@ class ModuliTable:
 class Request:
     """One energy + forces evaluation: a periodic orthorhombic box of water-like molecules."""
 
-    __slots__ = ("rid", "arrival", "coords", "charges", "sigma", "epsilon", "molecule", "box_lengths",
-                 "grid", "natoms", "t_submit", "t_done", "energies", "forces", "_L", "_K", "_mod")
+    __slots__ = ("rid", "arrival", "records", "box_lengths", "grid", "natoms", "t_submit", "t_done", "energies",
+                 "forces", "_L", "_K", "_mod")
 
     def __init__(self, rid, coords, charges, sigma, epsilon, molecule, box_lengths, grid, arrival=0.0):
         self.rid = rid
         self.arrival = float(arrival)
-        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
-        self.charges = np.ascontiguousarray(charges, dtype=np.float64)
-        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
-        self.epsilon = np.ascontiguousarray(epsilon, dtype=np.float64)
-        self.molecule = np.ascontiguousarray(molecule, dtype=np.int32)
+        n = len(charges)
+        # the atom records of the batch table: x, y, z, q, sigma, epsilon, molecule, 0 (float64)
+        rec = np.zeros((n, 8), dtype=np.float64)
+        rec[:, 0:3] = np.asarray(coords, dtype=np.float64).reshape(n, 3)
+        rec[:, 3] = charges
+        rec[:, 4] = sigma
+        rec[:, 5] = epsilon
+        rec[:, 6] = np.asarray(molecule, dtype=np.int64)
+        self.records = rec
         self.box_lengths = tuple(float(x) for x in box_lengths)
         self.grid = tuple(int(k) for k in grid)
-        self.natoms = len(self.charges)
+        self.natoms = n
         self.t_submit = self.t_done = None
         self.energies = self.forces = None
         self._L = (ctypes.c_double * 3)(*self.box_lengths)
@@ -118,46 +122,40 @@ class _Slot:
         self.index = index
         dev = eng.device
         c = eng.caps
-        self.stream = torch.cuda.Stream(device=dev)
         lib = L.load()
-        self.table = torch.empty(lib.tff_serve_table_bytes(c["max_systems"]), dtype=torch.uint8, pin_memory=True)
+        self.stream = torch.cuda.Stream(device=dev)
+        self.sptr = ctypes.c_void_p(self.stream.cuda_stream)
+        ms, na = c["max_systems"], c["max_atoms"]
+        self.table = torch.empty(lib.tff_serve_table_bytes(ms, na), dtype=torch.uint8, pin_memory=True)
         self.table_ptr = ctypes.c_void_p(self.table.data_ptr())
+        off = lib.tff_serve_atoms_offset(ms)
+        self.n_atoms = self.table.numpy()[off:].view(np.float64).reshape(na, 8)
         nbytes = lib.tff_serve_workspace_bytes(eng.cd, *eng.cap_args)
         if nbytes == 0:
             raise RuntimeError("ServingEngine: capacities rejected (max_k too large for one FFT tile's LDS)")
         self.ws = L.workspace(nbytes, dev)
-        na = c["max_atoms"]
-        f64 = dict(dtype=torch.float64)
-        self.h_in = torch.empty(6 * na, pin_memory=True, **f64)  # coords (3 na), q, sigma, epsilon
-        self.h_mol = torch.empty(na, dtype=torch.int32, pin_memory=True)
-        self.h_out = torch.empty(3 * na, pin_memory=True, **f64)
-        self.h_energy = torch.empty(5 * c["max_systems"], pin_memory=True, **f64)
-        self.d_in = torch.empty(6 * na, device=dev, **f64)
-        self.d_mol = torch.empty(na, dtype=torch.int32, device=dev)
-        self.d_out = torch.empty(3 * na, device=dev, **f64)
-        self.d_energy = torch.empty(5 * c["max_systems"], device=dev, **f64)
-        self.n_in = self.h_in.numpy()
-        self.n_mol = self.h_mol.numpy()
+        L.check(lib.tff_serve_workspace_init(eng.cd, *eng.cap_args, L.ptr(self.ws), self.ws.numel(), self.sptr),
+                "serve workspace init")
+        self.nout = 5 * ms + 3 * na
+        self.d_out = torch.empty(self.nout, dtype=torch.float64, device=dev)
+        self.h_out = torch.empty(self.nout, dtype=torch.float64, pin_memory=True)
         self.n_out = self.h_out.numpy()
-        self.n_energy = self.h_energy.numpy()
-        self.d_x = self.d_in[: 3 * na]
-        self.d_q = self.d_in[3 * na: 4 * na]
-        self.d_s = self.d_in[4 * na: 5 * na]
-        self.d_e = self.d_in[5 * na:]
         self.requests = []
         self.natoms = 0
         self.event = torch.cuda.Event()
         self.graph = None
-        self.bytes = (nbytes + self.d_in.numel() * 8 + self.d_mol.numel() * 4 + self.d_out.numel() * 8
-                      + self.d_energy.numel() * 8)
+        self.bytes = nbytes + self.nout * 8
 
     def launch_call(self):
         eng = self.eng
-        rc = L.load().tff_serve_energy_forces(
-            L.F64, eng.cd, L.ptr(self.d_x), L.ptr(self.d_q), L.ptr(self.d_s), L.ptr(self.d_e), L.ptr(self.d_mol),
-            eng.coul, eng.order, L.ptr(eng.moduli.table), *eng.cap_args, L.ptr(self.d_energy), L.ptr(self.d_out),
-            L.ptr(self.ws), self.ws.numel(), ctypes.c_void_p(self.stream.cuda_stream))
+        rc = L.load().tff_serve_energy_forces(eng.cd, eng.coul, eng.order, L.ptr(eng.moduli.table), *eng.cap_args,
+                                              L.ptr(self.d_out), L.ptr(self.ws), self.ws.numel(), self.sptr)
         L.check(rc, "serve energy+forces")
+
+    def upload(self):
+        rc = L.load().tff_serve_upload(self.table_ptr, self.eng.cd, *self.eng.cap_args, L.ptr(self.ws),
+                                       self.ws.numel(), self.sptr)
+        L.check(rc, "serve upload")
 
     def reset(self):
         L.load().tff_serve_table_reset(self.table_ptr)
@@ -167,8 +165,6 @@ class _Slot:
     def try_add(self, r):
         """Adds a request to the batch being formed; False when it does not fit."""
         eng = self.eng
-        if self.natoms + r.natoms > eng.caps["max_atoms"]:
-            return False
         if r._mod is None:
             if any(k not in eng.moduli.offsets for k in r.grid):
                 return False
@@ -178,42 +174,30 @@ class _Slot:
         if rc == L.SERVE_FULL:
             return False
         L.check(rc, "serve table")
-        na, a0, n = eng.caps["max_atoms"], self.natoms, r.natoms
-        self.n_in[3 * a0: 3 * (a0 + n)] = r.coords.reshape(-1)
-        self.n_in[3 * na + a0: 3 * na + a0 + n] = r.charges
-        self.n_in[4 * na + a0: 4 * na + a0 + n] = r.sigma
-        self.n_in[5 * na + a0: 5 * na + a0 + n] = r.epsilon
-        self.n_mol[a0: a0 + n] = r.molecule
+        self.n_atoms[self.natoms: self.natoms + r.natoms] = r.records
         self.requests.append(r)
-        self.natoms += n
+        self.natoms += r.natoms
         return True
 
     def launch(self):
-        """Uploads the batch, runs it (graph replay when captured) and queues the copies back."""
-        eng = self.eng
-        na, n, ns = eng.caps["max_atoms"], self.natoms, len(self.requests)
-        lib = L.load()
+        """Uploads the batch, runs it (graph replay when captured) and queues the copy back."""
+        n = 5 * self.eng.caps["max_systems"] + 3 * self.natoms
         with torch.cuda.stream(self.stream):
-            self.d_in[: 3 * n].copy_(self.h_in[: 3 * n], non_blocking=True)
-            for k in (3, 4, 5):
-                self.d_in[k * na: k * na + n].copy_(self.h_in[k * na: k * na + n], non_blocking=True)
-            self.d_mol[:n].copy_(self.h_mol[:n], non_blocking=True)
-            rc = lib.tff_serve_upload(self.table_ptr, eng.cd, *eng.cap_args, L.ptr(self.ws), self.ws.numel(),
-                                      ctypes.c_void_p(self.stream.cuda_stream))
-            L.check(rc, "serve upload")
+            # the upload stays outside the graph: its size follows the batch
+            self.upload()
             if self.graph is not None:
                 self.graph.replay()
             else:
                 self.launch_call()
-            self.h_out[: 3 * n].copy_(self.d_out[: 3 * n], non_blocking=True)
-            self.h_energy[: 5 * ns].copy_(self.d_energy[: 5 * ns], non_blocking=True)
+            self.h_out[:n].copy_(self.d_out[:n], non_blocking=True)
             self.event.record(self.stream)
 
     def finish(self, now):
+        f0 = 5 * self.eng.caps["max_systems"]
         a0 = 0
         for s, r in enumerate(self.requests):
-            r.energies = self.n_energy[5 * s: 5 * s + 5].copy()
-            r.forces = self.n_out[3 * a0: 3 * (a0 + r.natoms)].reshape(-1, 3).copy()
+            r.energies = self.n_out[5 * s: 5 * s + 5].copy()
+            r.forces = self.n_out[f0 + 3 * a0: f0 + 3 * (a0 + r.natoms)].reshape(-1, 3).copy()
             r.t_done = now
             a0 += r.natoms
         done = self.requests
@@ -224,13 +208,12 @@ class _Slot:
 class ServingEngine:
     """Continuous-batching evaluator (see the module docstring).
 
-    capacities per slot: max_systems, max_atoms, max_grid_points, max_k (largest grid edge);
-    the task and tile capacities follow from them.  precision "mixed" (float32 arithmetic,
+    capacities per slot: max_systems, max_atoms, max_grid_points, max_k (largest grid edge).  precision "mixed" (float32 arithmetic,
     int64 fixed-point grid, float64 sums) or "fp64".  Results: energies per request
     [real space, 0, reciprocal, self, total] (kJ/mol) and forces (kJ/mol/nm).
     """
 
-    def __init__(self, slots=2, max_systems=64, max_atoms=65536, max_grid_points=1 << 21, max_k=64,
+    def __init__(self, slots=2, max_systems=64, max_atoms=65536, max_grid_points=1 << 21, max_k=32,
                  cutoff=0.9, ewald_alpha_value=None, order=5, precision="mixed", coul=COULOMB_KJMOL,
                  use_graph=True, device="cuda"):
         if precision not in ("mixed", "fp64"):
@@ -243,12 +226,8 @@ class ServingEngine:
         self.alpha = float(ewald_alpha(cutoff) if ewald_alpha_value is None else ewald_alpha_value)
         self.order = int(order)
         self.coul = float(coul)
-        max_tasks = max_atoms // 64 + max_systems
-        # an FFT pass of a K^3 grid (K >= 9) has at most K^2 / 16 + 2 K tiles of 16 lines
-        max_tiles = max_grid_points // 144 + 2 * max_systems * max_k
-        self.caps = dict(max_systems=max_systems, max_tasks=max_tasks, max_tiles=max_tiles, max_k=max_k,
-                         max_grid_points=max_grid_points, max_atoms=max_atoms)
-        self.cap_args = (max_systems, max_tasks, max_tiles, max_k, max_grid_points)
+        self.caps = dict(max_systems=max_systems, max_atoms=max_atoms, max_k=max_k, max_grid_points=max_grid_points)
+        self.cap_args = (max_systems, max_atoms, max_k, max_grid_points)
         cdt = torch.float32 if precision == "mixed" else torch.float64
         self.moduli = ModuliTable(self.order, max_k, cdt, self.device)
         self.slots = [_Slot(self, i) for i in range(slots)]
@@ -265,8 +244,7 @@ class ServingEngine:
             s.reset()
             # an empty table: the captured launches read the batch from the workspace at replay
             with torch.cuda.stream(s.stream):
-                L.check(L.load().tff_serve_upload(s.table_ptr, self.cd, *self.cap_args, L.ptr(s.ws), s.ws.numel(),
-                                                  ctypes.c_void_p(s.stream.cuda_stream)), "serve upload")
+                s.upload()
                 s.launch_call()
             s.stream.synchronize()
             g = torch.cuda.CUDAGraph()
@@ -339,13 +317,12 @@ class ServingEngine:
                 busy.append(s)
                 nb += 1
                 progressed = True
-            if not progressed:
-                if busy:
-                    time.sleep(20e-6)
-                elif nxt < len(order) and realtime:
-                    wait = order[nxt].arrival - (time.perf_counter() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 1e-3))
+            if not progressed and not busy and nxt < len(order) and realtime:
+                # idle: sleep only when the next arrival is far (the kernel's sleep overshoots by
+                # tens of microseconds); while batches are in flight the loop polls without sleeping
+                wait = order[nxt].arrival - (time.perf_counter() - t0)
+                if wait > 2e-4:
+                    time.sleep(min(wait - 1e-4, 1e-3))
         return order, nb
 
 
