@@ -15,7 +15,10 @@ Paths measured (same inputs):
            explicit (P, 2) pair list + Nonbonded(ewald) + PME, autograd forces):
            the plain FP64 drop-in the speed-up is quoted against.
 Errors are stated against one float64 CPU oracle evaluation, which is also the
-timed CPU baseline.  Prints one JSON line.  --gpus above 1 is refused.
+timed CPU baseline.  The line also carries the batched-serving results ("serving":
+p50/p99 latency of 4096 Poisson requests at 2000 req/s of 20-1000-water boxes,
+batched against one system per launch, and saturation throughput).  Prints one
+JSON line.  --gpus above 1 is refused.
 """
 
 import argparse
@@ -56,6 +59,9 @@ def parse():
     ap.add_argument("--serving", action="store_true",
                     help="serve 4096 Poisson requests (20-1000 waters) batched and one system per launch instead")
     ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--no-serving", action="store_true", help="leave the serving results out of the main line")
+    ap.add_argument("--serving-sizes", default="", metavar="N,N,...",
+                    help="only time single-system serving batches of these water counts")
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (to calibrate HBM byte counters)")
@@ -224,7 +230,7 @@ def differentiation_bench(args, w, dev):
     }))
 
 
-def serving_bench(args, dev):
+def serving_results(args, dev):
     """Request serving: args.requests water boxes of 20-1000 waters (log-uniform, 1 g/cm^3, rc 0.9 nm,
     PME spacing <= 0.1 nm rounded to 2^a 3^b 5^c, order 5, mixed precision) arriving as a Poisson
     stream at args.rate per second.  The same stream is served by continuous batching (graph-replayed
@@ -233,43 +239,79 @@ def serving_bench(args, dev):
     the host (energies and forces copied back) minus arrival."""
     from torchff.serving import ServingEngine, latency_stats, request_stream
 
-    def stream():
-        return request_stream(args.requests, rate=args.rate, seed=11)
-
     modes = {"batched": dict(slots=2, max_systems=64, max_atoms=1 << 16, max_grid_points=1 << 21),
              "one_per_launch": dict(slots=2, max_systems=1, max_atoms=3072, max_grid_points=1 << 15)}
+    # the stated rate, ten times that rate, and everything queued at t = 0 (saturation)
+    loads = {"poisson": args.rate, "poisson_x10": 10.0 * args.rate, "saturated": None}
     res, results, hbm = {}, {}, {}
     for name, caps in modes.items():
         eng = ServingEngine(max_k=32, precision="mixed", device=dev, **caps)
         hbm[name] = eng.hbm_bytes
         eng.evaluate(request_stream(64, seed=3))  # warm-up
-        for realtime in (True, False):
-            reqs = stream()
-            done, nb = eng.serve(reqs, realtime=realtime)
+        for load, rate in loads.items():
+            reqs = request_stream(args.requests, rate=rate or args.rate, seed=11)
+            done, nb = eng.serve(reqs, realtime=rate is not None)
             st = latency_stats(done)
+            st["offered_req_per_s"] = rate
             st["batches"] = nb
             st["systems_per_batch"] = len(done) / nb
-            res[f"{name}_{'poisson' if realtime else 'saturated'}"] = st
-            if realtime:
+            res.setdefault(load, {})[name] = st
+            if load == "poisson":
                 results[name] = {r.rid: (r.energies, r.forces) for r in done}
     same = all(np.array_equal(results["batched"][k][0], results["one_per_launch"][k][0]) and
                np.array_equal(results["batched"][k][1], results["one_per_launch"][k][1]) for k in results["batched"])
-    b, o = res["batched_poisson"], res["one_per_launch_poisson"]
+    return {
+        "workload": f"{args.requests} water boxes, 20-1000 waters log-uniform at 1 g/cm^3, Poisson arrivals at "
+                    f"{args.rate:.0f} req/s, rc 0.9 nm, PME spacing <= 0.1 nm (2^a 3^b 5^c), order 5, mixed "
+                    "precision, all periodic images in range",
+        "latency_is": "host-side completion (energies and forces copied back) minus arrival",
+        **res,
+        "p99_ratio_one_per_launch_over_batched": {
+            k: res[k]["one_per_launch"]["p99_ms"] / res[k]["batched"]["p99_ms"] for k in ("poisson", "poisson_x10")},
+        "saturated_speedup_vs_one_per_launch": res["saturated"]["batched"]["evals_per_s"]
+        / res["saturated"]["one_per_launch"]["evals_per_s"],
+        "results_identical_batched_vs_alone": bool(same),
+        "slot_hbm_bytes": hbm,
+    }
+
+
+def serving_sizes(args, dev):
+    """Single-system batches of the listed water counts (20 each after 3 warm-up): device time of
+    the upload + launches + read-back per batch from events on the slot's stream."""
+    from torchff.serving import ServingEngine, water_request
+    eng = ServingEngine(slots=1, max_systems=64, max_atoms=1 << 16, max_grid_points=1 << 21, max_k=32, device=dev)
+    s = eng.slots[0]
+    out = {}
+    for nw in (int(x) for x in args.serving_sizes.split(",")):
+        dev_ms = []
+        for rep in range(23):
+            r = water_request(0, nw, 1)
+            assert s.try_add(r)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s.stream)
+            s.launch()
+            e1.record(s.stream)
+            e1.synchronize()
+            s.finish(0.0)
+            if rep >= 3:
+                dev_ms.append(e0.elapsed_time(e1))
+        out[f"{nw}_waters"] = {"atoms": 3 * nw, "grid": r.grid[0], "device_ms_median": float(np.median(dev_ms))}
+    print(json.dumps({"serving_single_system_batches": out}))
+
+
+def serving_bench(args, dev):
+    r = serving_results(args, dev)
+    sat = r["saturated"]["batched"]["evals_per_s"]
     print(json.dumps({
         "metric": f"serving: p50/p99 latency and evals/s, {args.requests} Poisson requests at {args.rate:.0f} req/s, "
                   "20-1000 waters",
-        "value": res["batched_saturated"]["evals_per_s"], "unit": "evals/s", "n_gpus": 1,
-        "steps": args.requests, "warmup": 64, "ms_per_step": 1e3 / res["batched_saturated"]["evals_per_s"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i64", "data": "synthetic",
-        "config": {"workload": f"{args.requests} water boxes, 20-1000 waters log-uniform at 1 g/cm^3, rc 0.9 nm, "
-                               "PME spacing <= 0.1 nm (2^a 3^b 5^c), order 5, all periodic images in range",
-                   "path": "tff_serve_energy_forces, continuous batching over 2 graph-replayed slots",
-                   "value_is": "saturation throughput of the batched engine (all requests queued at t=0)"},
-        "poisson": {"batched": b, "one_per_launch": o},
-        "saturated": {"batched": res["batched_saturated"], "one_per_launch": res["one_per_launch_saturated"]},
-        "p99_speedup_vs_one_per_launch": o["p99_ms"] / b["p99_ms"],
-        "results_identical_batched_vs_alone": bool(same),
-        "slot_hbm_bytes": hbm,
+        "value": sat, "unit": "evals/s", "n_gpus": 1, "steps": args.requests, "warmup": 64,
+        "ms_per_step": 1e3 / sat, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+i64", "data": "synthetic",
+        "config": {"workload": r["workload"], "path": "tff_serve_energy_forces, continuous batching over 2 "
+                   "graph-replayed slots", "value_is": "saturation throughput of the batched engine (all "
+                   "requests queued at t=0)"},
+        **{k: v for k, v in r.items() if k != "workload"},
     }))
 
 
@@ -301,6 +343,9 @@ def main():
     dev = "cuda:0"
     if args.serving:
         serving_bench(args, dev)
+        return
+    if args.serving_sizes:
+        serving_sizes(args, dev)
         return
     only = set(args.only.split(",")) if args.only else {"mixed", "fp64", "dropin"}
     w = make_workload(args.workload)
@@ -382,6 +427,8 @@ def main():
                     "GB/s": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9,
                     "frac_of_hbm_peak": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
                for st, b in sb.items()}}
+    if head == "mixed" and not args.no_serving:
+        out["serving"] = serving_results(args, dev)
     if not args.no_cpu_baseline:
         from oracle import composite
         threads = os.cpu_count() or 1

@@ -1100,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
   __shared__ int s_list[BRICK_MAXC];
   __shared__ V4<TC> s_pos[SPREAD_CHUNK * CL];
   __shared__ int s_slot[DIPOLE ? SPREAD_CHUNK * CL : 1];
-  __shared__ int s_coff[SPREAD_MAXCOL + 1], s_ccs[SPREAD_MAXCOL];
+  __shared__ int s_coff[2 * SPREAD_MAXCOL + 1], s_ccs[2 * SPREAD_MAXCOL], s_cln[2 * SPREAD_MAXCOL];
   __shared__ int s_n, s_na;
   const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
   const int nbx = (K0 + BR - 1) / BR, nby = (K1 + BR - 1) / BR, nbz = (K2 + BR - 1) / BR;
@@ -1124,30 +1124,83 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
     nc[a] = ch - cl + 1 < g.ncol[a] ? ch - cl + 1 : g.ncol[a];
   }
   const int ncand_cols = nc[0] * nc[1];
-  // the candidate columns' cluster ranges, scanned into one flat range (columns in groups of
-  // SPREAD_MAXCOL), then every candidate cluster tested in one pass
+  // the brick's z window (stencil reach plus two grid points), as up to two intervals of [0, L)
+  const double zw_lo = (B0[2] - P / 2 - 2) * g.L[2] / pg.K[2];
+  const double zw_hi = (B0[2] + Bn[2] + P / 2 + 2) * g.L[2] / pg.K[2];
+  const bool zall = zw_hi - zw_lo >= g.L[2];
+  const float za0 = zw_lo < 0 ? 0.f : (float)zw_lo, zb0 = zw_hi > g.L[2] ? (float)g.L[2] : (float)zw_hi;
+  const bool zwrap = !zall && (zw_lo < 0 || zw_hi > g.L[2]);
+  const float za1 = zw_lo < 0 ? (float)(zw_lo + g.L[2]) : 0.f;          // second interval [za1, zb1]
+  const float zb1 = zw_lo < 0 ? (float)g.L[2] : (float)(zw_hi - g.L[2]);
+  // the candidate columns' cluster sub-ranges inside that window (clusters of a column are z
+  // ordered: binary searches, one thread per column), scanned into one flat range (columns in
+  // groups of SPREAD_MAXCOL), then every candidate cluster tested in one pass
   for (int g0 = 0; g0 < ncand_cols; g0 += SPREAD_MAXCOL) {
     const int ngc = ncand_cols - g0 < SPREAD_MAXCOL ? ncand_cols - g0 : SPREAD_MAXCOL;
-    if (threadIdx.x == 0) {
-      int run = 0;
-      for (int t = 0; t < ngc; ++t) {
-        const int cc = g0 + t;
-        int xx = (c0[0] + cc / nc[1]) % g.ncol[0];
-        if (xx < 0) xx += g.ncol[0];
-        int yy = (c0[1] + cc % nc[1]) % g.ncol[1];
-        if (yy < 0) yy += g.ncol[1];
-        const int col = xx * g.ncol[1] + yy;
-        const int cs = col_cl_start[col], ce = col_cl_start[col + 1];
-        s_coff[t] = run;
-        s_ccs[t] = cs;
-        run += ce - cs;
+    if (threadIdx.x < ngc) {
+      const int cc = g0 + threadIdx.x;
+      int xx = (c0[0] + cc / nc[1]) % g.ncol[0];
+      if (xx < 0) xx += g.ncol[0];
+      int yy = (c0[1] + cc % nc[1]) % g.ncol[1];
+      if (yy < 0) yy += g.ncol[1];
+      const int col = xx * g.ncol[1] + yy;
+      const int cs = col_cl_start[col], ce = col_cl_start[col + 1];
+      // first cluster with z max >= a, last cluster with z min <= b (both monotone in the column)
+      auto first_ge = [&](float a) {
+        int lo = cs, hi = ce;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (bb_c[mid].z + bb_h[mid].z >= a) hi = mid; else lo = mid + 1;
+        }
+        return lo;
+      };
+      auto last_le = [&](float b) {
+        int lo = cs - 1, hi = ce - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (bb_c[mid].z - bb_h[mid].z <= b) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+      };
+      int r0s = cs, r0e = ce, r1s = 0, r1e = 0;  // [start, end) of up to two ranges
+      if (!zall && cs < ce) {
+        if (!zwrap) {
+          r0s = first_ge(za0);
+          r0e = last_le(zb0) + 1;
+        } else {
+          // [0, b] at the column's start, [a, L) at its end; merged when they meet
+          const int l0 = last_le(zw_lo < 0 ? zb0 : zb1) + 1;
+          const int f1 = first_ge(zw_lo < 0 ? za1 : za0);
+          if (f1 <= l0) {
+            r0s = cs;
+            r0e = ce;
+          } else {
+            r0s = cs;
+            r0e = l0;
+            r1s = f1;
+            r1e = ce;
+          }
+        }
       }
-      s_coff[ngc] = run;
+      s_ccs[2 * threadIdx.x] = r0s;
+      s_cln[2 * threadIdx.x] = r0e > r0s ? r0e - r0s : 0;
+      s_ccs[2 * threadIdx.x + 1] = r1s;
+      s_cln[2 * threadIdx.x + 1] = r1e > r1s ? r1e - r1s : 0;
     }
     __syncthreads();
-    const int total = s_coff[ngc];
+    if (threadIdx.x == 0) {
+      int run = 0;
+      for (int t = 0; t < 2 * ngc; ++t) {
+        s_coff[t] = run;
+        run += s_cln[t];
+      }
+      s_coff[2 * ngc] = run;
+    }
+    __syncthreads();
+    const int nent = 2 * ngc;
+    const int total = s_coff[nent];
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
-      int lo = 0, hi = ngc - 1;
+      int lo = 0, hi = nent - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (s_coff[mid] <= k) lo = mid; else hi = mid - 1;
