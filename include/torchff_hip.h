@@ -115,6 +115,21 @@ int tff_pme_gather(int dtype, const void* phi_grid, const void* coords, const vo
                    const double* box_host, int K0, int K1, int K2, int order, double alpha, double scale,
                    void* epot_out, void* dcoords_out, tff_stream_t stream);
 
+/* ------------------------------------------------------- direct Ewald */
+size_t tff_ewald_workspace_bytes(int dtype, long long natoms, int kmax);
+
+/* Replaces torch.ops.torchff.ewald_long_range and ewald_long_range_all for point charges
+ * (rank 0; csrc/ewald/ewald_interface.cpp:6-11, csrc/ewald/ewald.cu:9-171,304-420; Python
+ * torchff/ewald.py:60-110): the FFT-free reciprocal sum over the half-space integer vectors
+ * |m_a| <= kmax (1 <= kmax <= 31), k = B^-1 m, G = exp(-pi^2 k^2/alpha^2)/k^2.  energy_out (device,
+ * 1 double) = 1/(pi V) sum G |S|^2 - alpha/sqrt(pi) sum q^2 (Gaussian units); potential_out
+ * (dtype, N) = dE/dq including the self term; forces_out (dtype, N x 3) = -dE/dr.  Either of the
+ * last two may be NULL.  k = B^-1 m is the reciprocal lattice of row-major lattice vectors; the
+ * reference forms m B^-1, the same for orthorhombic boxes. */
+int tff_ewald(int dtype, const void* coords, const void* charges, long long natoms, const double* box_host, int kmax,
+              double alpha, double* energy_out, void* potential_out, void* forces_out, void* workspace,
+              size_t workspace_bytes, tff_stream_t stream);
+
 /* ------------------------------------------------- fused energy + forces */
 #define TFF_FUSED_PAIR_OVERFLOW 1  /* cluster-pair capacity exceeded: results incomplete */
 #define TFF_FUSED_EXCL_OVERFLOW 2  /* an atom has more than 8 exclusion partners */

@@ -17,4 +17,5 @@ from .nonbonded import (  # noqa: F401
 from .multipoles import MultipolarInteraction  # noqa: F401
 from .vdw import Vdw, compute_lennard_jones_energy  # noqa: F401
 from .pme import PME  # noqa: F401
+from .ewald import Ewald, ewald_long_range, ewald_long_range_with_fields  # noqa: F401
 from . import bsplines  # noqa: F401

@@ -273,3 +273,87 @@ def _pme_long_range(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, o
 
 
 _auto.impl("pme_long_range", _pme_long_range)
+
+
+# ---------------------------------------------------------------- direct Ewald
+_def.define("ewald_long_range(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar k_max, "
+            "Scalar alpha) -> Tensor")
+_def.define("ewald_long_range_all(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar k_max, "
+            "Scalar alpha) -> (Tensor, Tensor, Tensor)")
+
+
+def _ewald_call(coords, box, q, kmax, alpha):
+    L.require_cuda(coords, box, q)
+    lib = L.load()
+    c = coords.detach().contiguous()
+    dt = c.dtype
+    qq = q.detach().to(dt).contiguous()
+    n = c.shape[0]
+    bh = L.box_host(box)
+    ws = L.workspace(lib.tff_ewald_workspace_bytes(L.dtype_code(c), n, int(kmax)), c.device)
+    e = torch.empty((), dtype=torch.float64, device=c.device)
+    pot = torch.empty(n, dtype=dt, device=c.device)
+    forces = torch.empty_like(c)
+    rc = lib.tff_ewald(L.dtype_code(c), L.ptr(c), L.ptr(qq), n, bh, int(kmax), float(alpha), L.ptr(e), L.ptr(pot),
+                       L.ptr(forces), L.ptr(ws), ws.numel(), L.stream_of(c))
+    L.check(rc, "ewald_long_range")
+    return e.to(dt), pot, forces
+
+
+class _EwaldEnergy(torch.autograd.Function):
+    """Energy, with the reference's backward: dE/dcoords = -forces, dE/dq = potential, both formed
+    in the forward and scaled by the incoming gradient (csrc/ewald/ewald.cu:123-171)."""
+
+    @staticmethod
+    def forward(ctx, coords, box, q, kmax, alpha):
+        e, pot, forces = _ewald_call(coords, box, q, kmax, alpha)
+        ctx.save_for_backward(pot, forces)
+        return e
+
+    @staticmethod
+    def backward(ctx, ge):
+        pot, forces = ctx.saved_tensors
+        s = ge.reshape(())
+        return -forces * s, None, pot * s, None, None
+
+
+class _EwaldAll(torch.autograd.Function):
+    """(energy, potential, field) for point charges; the field of a rank-0 evaluation is zero, as
+    the reference returns it (torchff/ewald.py:103-104); gradients flow from the energy only, as
+    in the reference's rank-0 backward (csrc/ewald/ewald.cu:450-470)."""
+
+    @staticmethod
+    def forward(ctx, coords, box, q, kmax, alpha):
+        e, pot, forces = _ewald_call(coords, box, q, kmax, alpha)
+        ctx.save_for_backward(pot, forces)
+        ctx.mark_non_differentiable(pot)
+        field = torch.zeros_like(forces)
+        ctx.mark_non_differentiable(field)
+        return e, pot, field
+
+    @staticmethod
+    def backward(ctx, ge, gp, gf):
+        pot, forces = ctx.saved_tensors
+        if ge is None:
+            return None, None, None, None, None
+        s = ge.reshape(())
+        return -forces * s, None, pot * s, None, None
+
+
+def _check_rank0(p, t, name):
+    if p is not None or t is not None:
+        raise NotImplementedError(f"{name}: only rank 0 (point charges) is provided by the gfx950 build")
+
+
+def _ewald_long_range(coords, box, q, p, t, k_max, alpha):
+    _check_rank0(p, t, "ewald_long_range")
+    return _EwaldEnergy.apply(coords, box, q, int(k_max), float(alpha))
+
+
+def _ewald_long_range_all(coords, box, q, p, t, k_max, alpha):
+    _check_rank0(p, t, "ewald_long_range_all")
+    return _EwaldAll.apply(coords, box, q, int(k_max), float(alpha))
+
+
+_auto.impl("ewald_long_range", _ewald_long_range)
+_auto.impl("ewald_long_range_all", _ewald_long_range_all)

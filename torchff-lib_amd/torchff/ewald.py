@@ -1,0 +1,44 @@
+"""Direct (FFT-free) Ewald reciprocal sum (drop-in for torchff/ewald.py of the reference).
+
+Same functions and module as the reference (torchff/ewald.py:9-59): ``ewald_long_range``,
+``ewald_long_range_with_fields`` and ``Ewald(alpha, kmax, rank, use_customized_ops=True,
+return_fields=True)``, backed by ``tff_ewald`` (gfx950).  Rank 0 (point charges) only; the
+pure-PyTorch path (``use_customized_ops=False``) is not shipped: the float64 restatement lives in
+the oracle used by the tests.
+"""
+
+import torch
+import torch.nn as nn
+
+from . import _ops  # noqa: F401  (registers torch.ops.torchff.ewald_*)
+from ._common import require_custom
+
+
+@torch._dynamo.disable
+def ewald_long_range(coords: torch.Tensor, box: torch.Tensor, q: torch.Tensor, p, t, kmax: int, alpha: float):
+    return torch.ops.torchff.ewald_long_range(coords, box, q, p, t, kmax, alpha)
+
+
+@torch._dynamo.disable
+def ewald_long_range_with_fields(coords: torch.Tensor, box: torch.Tensor, q: torch.Tensor, p, t, kmax: int,
+                                 alpha: float):
+    return torch.ops.torchff.ewald_long_range_all(coords, box, q, p, t, kmax, alpha)
+
+
+class Ewald(nn.Module):
+    def __init__(self, alpha: float, kmax: int, rank: int, use_customized_ops: bool = True,
+                 return_fields: bool = True):
+        super().__init__()
+        require_custom(use_customized_ops, "Ewald")
+        if rank != 0:
+            raise NotImplementedError("Ewald: only rank 0 (point charges) is provided by the gfx950 build")
+        self.alpha = float(alpha)
+        self.kmax = int(kmax)
+        self.rank = rank
+        self.return_fields = return_fields
+        self.use_customized_ops = use_customized_ops
+
+    def forward(self, coords, box, q, p=None, t=None):
+        if self.return_fields:
+            return ewald_long_range_with_fields(coords, box, q, p, t, self.kmax, self.alpha)
+        return ewald_long_range(coords, box, q, p, t, self.kmax, self.alpha)
