@@ -94,3 +94,23 @@ def test_bad_grid_raises():
     x = torch.zeros(3, 3, device="cuda", dtype=torch.float64)
     with pytest.raises(RuntimeError, match="2\\^a 3\\^b 5\\^c"):
         m(x, torch.eye(3, dtype=torch.float64, device="cuda"), torch.zeros(3, dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_pme_return_fields(dtype):
+    # pme_long_range_all: the same energy, the per-atom potential dE/dq (with the self term) and a
+    # zero field for point charges; gradients through the energy
+    import torchff
+    w = make_workload("water900")
+    E_ref, gx_ref, gq_ref = _ref(w["coords"], w["box"], w["charges"], 24, w["alpha"], 6)
+    m = torchff.PME(w["alpha"], 24, 0, return_fields=True).to(device="cuda", dtype=dtype)
+    x = torch.tensor(w["coords"], dtype=dtype, device="cuda", requires_grad=True)
+    q = torch.tensor(w["charges"], dtype=dtype, device="cuda", requires_grad=True)
+    E, pot, field = m(x, torch.tensor(w["box"], dtype=dtype, device="cuda"), q)
+    te, tf = TOL[dtype]
+    assert abs(float(E) - E_ref) < te * abs(E_ref)
+    assert _rms_rel(pot.double().cpu().numpy(), gq_ref) < tf
+    assert field.shape == (w["natoms"], 3) and float(field.abs().max()) == 0.0
+    E.backward()
+    assert _rms_rel(x.grad.double().cpu().numpy(), gx_ref) < tf
+    assert _rms_rel(q.grad.double().cpu().numpy(), gq_ref) < tf

@@ -911,7 +911,7 @@ __device__ __forceinline__ void pair_grad_terms(TC r2, bool excl, bool lj_on, TC
 
 // One wave per workgroup over a contiguous run of the tile list (as the forward pair kernel);
 // one i-group at a time (lane = (ii, jj) holds i-slot gi * 8 + ii against j-slot gj * 8 + jj),
-// minimum image per pair.
+// the tile's periodic shift (minimum image per pair on the tiles flagged for it).
 template <typename TC>
 __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict__ cpairs,
                                                          const int* __restrict__ ncp_ptr, long long cap,
@@ -937,14 +937,24 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
     const unsigned mask = info & 0xffffu;
     if (mask == 0) continue;
     const bool upper = ci != cj || jj > ii;
+    // tiles flagged for per-pair minimum image keep it; the others take the tile's periodic shift
+    const bool minimg = (info & INFO_MINIMG) != 0u;
+    const TC shx = minimg ? (TC)0 : ta.L[0] * (TC)((int)((info >> 16) & 3u) - 1);
+    const TC shy = minimg ? (TC)0 : ta.L[1] * (TC)((int)((info >> 18) & 3u) - 1);
+    const TC shz = minimg ? (TC)0 : ta.L[2] * (TC)((int)((info >> 20) & 3u) - 1);
     V4<TC> pj[4], vj[4];
     V2<TC> lJ[4];
+    TC ej_inv2[4];  // 1 / (2 eps_j)
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int sl = cj * CL + g * 8 + jj;
       pj[g] = xyzq[sl];
+      pj[g].x += shx;
+      pj[g].y += shy;
+      pj[g].z += shz;
       lJ[g] = lj[sl];
       vj[g] = vsl[sl];
+      ej_inv2[g] = lJ[g].y > (TC)0 ? (TC)2 / (lJ[g].y * lJ[g].y) : (TC)0;
     }
     TC aj[4][GN];
 #pragma unroll
@@ -975,9 +985,11 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
       for (int gj = 0; gj < 4; ++gj) {
         if (!((mask >> (gi * 4 + gj)) & 1u)) continue;
         TC dx = pi.x - pj[gj].x, dy = pi.y - pj[gj].y, dz = pi.z - pj[gj].z;
-        dx -= ta.L[0] * rint(dx * ta.invL[0]);
-        dy -= ta.L[1] * rint(dy * ta.invL[1]);
-        dz -= ta.L[2] * rint(dz * ta.invL[2]);
+        if (minimg) {  // uniform over the wave
+          dx -= ta.L[0] * rint(dx * ta.invL[0]);
+          dy -= ta.L[1] * rint(dy * ta.invL[1]);
+          dz -= ta.L[2] * rint(dz * ta.invL[2]);
+        }
         TC r2 = dx * dx + dy * dy + dz * dz;
         bool ok = r2 <= ta.rc2 && pi.x < (TC)PAD_LIMIT && pj[gj].x < (TC)PAD_LIMIT;
         if (gi == gj) ok = ok && upper;
@@ -1003,9 +1015,8 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
         ai[4] += ts;
         aj[gj][4] += ts;
         const TC te = okf * (gE * phi_lj - sdv * f_lj);
-        const TC ej_inv2 = lJ[gj].y > (TC)0 ? (TC)2 / (lJ[gj].y * lJ[gj].y) : (TC)0;
         ai[5] += te * ei_inv2;
-        aj[gj][5] += te * ej_inv2;
+        aj[gj][5] += te * ej_inv2[gj];
         (void)phi_c;
       }
       // i-slot sums over the 8 lanes sharing ii; lane jj < GN writes value jj

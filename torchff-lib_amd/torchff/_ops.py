@@ -221,31 +221,37 @@ def _grid_dims(K):
     return (int(K),) * 3
 
 
+def _pme_forward(coords, box, q, K, alpha, xm, ym, zm, order):
+    """Energy and the potential grid; returns (energy, coords, charges, phi, (K0, K1, K2, order, alpha))."""
+    L.require_cuda(coords, box, q, xm, ym, zm)
+    lib = L.load()
+    dt = coords.dtype
+    K0, K1, K2 = _grid_dims(K)
+    c = coords.detach().contiguous()
+    qq = q.detach().to(dt).contiguous()
+    xm_, ym_, zm_ = (m.detach().to(dt).contiguous() for m in (xm, ym, zm))
+    if xm_.numel() != K0 or ym_.numel() != K1 or zm_.numel() not in (K2, K2 // 2 + 1):
+        raise RuntimeError("pme_long_range: moduli lengths must be (K, K, K//2+1) or (K, K, K)")
+    zhalf = int(zm_.numel() != K2 or K2 == 1)
+    bh = L.box_host(box)
+    n = c.shape[0]
+    energy = torch.empty((), dtype=dt, device=c.device)
+    phi = torch.empty((K0, K1, K2), dtype=dt, device=c.device)
+    ws = L.workspace(lib.tff_pme_workspace_bytes(L.dtype_code(c), n, K0, K1, K2), c.device)
+    rc = lib.tff_pme_energy(L.dtype_code(c), L.ptr(c), L.ptr(qq), n, bh, K0, K1, K2, int(order), float(alpha),
+                            L.ptr(xm_), L.ptr(ym_), L.ptr(zm_), zhalf, 1.0, L.ptr(energy), L.ptr(phi),
+                            L.ptr(ws), ws.numel(), L.stream_of(c))
+    L.check(rc, "pme_long_range")
+    return energy, c, qq, phi, (K0, K1, K2, int(order), float(alpha))
+
+
 class _PmeEnergy(torch.autograd.Function):
     @staticmethod
     def forward(ctx, coords, box, q, K, alpha, xm, ym, zm, order):
-        L.require_cuda(coords, box, q, xm, ym, zm)
-        lib = L.load()
-        dt = coords.dtype
-        K0, K1, K2 = _grid_dims(K)
-        c = coords.detach().contiguous()
-        qq = q.detach().to(dt).contiguous()
-        xm_, ym_, zm_ = (m.detach().to(dt).contiguous() for m in (xm, ym, zm))
-        if xm_.numel() != K0 or ym_.numel() != K1 or zm_.numel() not in (K2, K2 // 2 + 1):
-            raise RuntimeError("pme_long_range: moduli lengths must be (K, K, K//2+1) or (K, K, K)")
-        zhalf = int(zm_.numel() != K2 or K2 == 1)
-        bh = L.box_host(box)
-        n = c.shape[0]
-        energy = torch.empty((), dtype=dt, device=c.device)
-        phi = torch.empty((K0, K1, K2), dtype=dt, device=c.device)
-        ws = L.workspace(lib.tff_pme_workspace_bytes(L.dtype_code(c), n, K0, K1, K2), c.device)
-        rc = lib.tff_pme_energy(L.dtype_code(c), L.ptr(c), L.ptr(qq), n, bh, K0, K1, K2, int(order), float(alpha),
-                                L.ptr(xm_), L.ptr(ym_), L.ptr(zm_), zhalf, 1.0, L.ptr(energy), L.ptr(phi),
-                                L.ptr(ws), ws.numel(), L.stream_of(c))
-        L.check(rc, "pme_long_range")
+        energy, c, qq, phi, meta = _pme_forward(coords, box, q, K, alpha, xm, ym, zm, order)
         ctx.save_for_backward(c, qq, phi)
         ctx.box = box
-        ctx.meta = (K0, K1, K2, int(order), float(alpha))
+        ctx.meta = meta
         ctx.need = (coords.requires_grad, q.requires_grad)
         return energy
 
@@ -273,6 +279,49 @@ def _pme_long_range(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, o
 
 
 _auto.impl("pme_long_range", _pme_long_range)
+
+_def.define("pme_long_range_all(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar K, Scalar alpha, "
+            "Tensor xmoduli, Tensor ymoduli, Tensor zmoduli, int order=6) -> (Tensor, Tensor, Tensor)")
+
+
+class _PmeAll(torch.autograd.Function):
+    """(energy, potential, field) for point charges (csrc/pme/pme.cu:251-414): the potential is
+    dE/dq including the self term, from the gather of the same forward call; the field of a
+    rank-0 evaluation is zero, as the reference returns it (its interpolation writes the field
+    for ranks >= 1 only, csrc/pme/kernels.cuh:407-411); gradients flow from the energy."""
+
+    @staticmethod
+    def forward(ctx, coords, box, q, K, alpha, xm, ym, zm, order):
+        e, c, qq, phi, meta = _pme_forward(coords, box, q, K, alpha, xm, ym, zm, order)
+        K0, K1, K2, order_, alpha_ = meta
+        lib = L.load()
+        n = c.shape[0]
+        pot = torch.empty_like(qq)
+        dc = torch.empty_like(c)
+        rc = lib.tff_pme_gather(L.dtype_code(c), L.ptr(phi), L.ptr(c), L.ptr(qq), n, L.box_host(box), K0, K1, K2,
+                                order_, alpha_, 1.0, L.ptr(pot), L.ptr(dc), L.stream_of(c))
+        L.check(rc, "pme_long_range_all")
+        ctx.save_for_backward(pot, dc)
+        field = torch.zeros_like(c)
+        ctx.mark_non_differentiable(pot, field)
+        return e, pot, field
+
+    @staticmethod
+    def backward(ctx, ge, gp, gf):
+        pot, dc = ctx.saved_tensors
+        if ge is None:
+            return (None,) * 9
+        s = ge.reshape(())
+        return dc * s, None, pot * s, None, None, None, None, None, None
+
+
+def _pme_long_range_all(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order=6):
+    if p is not None or t is not None:
+        raise NotImplementedError("pme_long_range_all: only rank 0 (point charges) is provided by the gfx950 build")
+    return _PmeAll.apply(coords, box, q, int(K), float(alpha), xmoduli, ymoduli, zmoduli, int(order))
+
+
+_auto.impl("pme_long_range_all", _pme_long_range_all)
 
 
 # ---------------------------------------------------------------- direct Ewald

@@ -4,7 +4,9 @@
 (torchff/pme.py:13-127) and adds ``order`` (B-spline order, default 6 as in
 the reference; 5 for the 96k- and 999k-atom workloads).  ``forward`` returns
 the reciprocal-space plus self energy in Gaussian units (multiply by the
-Coulomb constant), differentiable in coords and q.
+Coulomb constant), differentiable in coords and q; with ``return_fields=True``
+it returns (energy, per-atom potential incl. the self term, field), the field
+being zero for point charges as in the reference (``pme_long_range_all``).
 """
 
 from typing import Optional
@@ -23,8 +25,6 @@ class PME(nn.Module):
         super().__init__()
         if rank != 0:
             raise NotImplementedError("PME: only rank 0 (point charges) is provided by the gfx950 build")
-        if return_fields:
-            raise NotImplementedError("PME: return_fields is not provided by the gfx950 build")
         require_custom(use_customized_ops, "PME")
         self.alpha = float(alpha)
         self.max_hkl = int(max_hkl)
@@ -40,5 +40,9 @@ class PME(nn.Module):
     @torch._dynamo.disable
     def forward(self, coords: torch.Tensor, box: torch.Tensor, q: torch.Tensor, p: Optional[torch.Tensor] = None,
                 t: Optional[torch.Tensor] = None):
+        if self.return_fields:
+            # (energy, potential incl. the self term, field: zero for point charges)
+            return torch.ops.torchff.pme_long_range_all(coords, box, q, p, t, self.max_hkl, self.alpha,
+                                                        self.xmoduli, self.ymoduli, self.zmoduli, self.order)
         return torch.ops.torchff.pme_long_range(coords, box, q, p, t, self.max_hkl, self.alpha, self.xmoduli,
                                                 self.ymoduli, self.zmoduli, self.order)
