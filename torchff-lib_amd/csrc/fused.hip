@@ -669,11 +669,15 @@ __device__ __forceinline__ void tile_steps(unsigned mask, bool upper, const V4<T
     bool ok = r2 <= ta.rc2;
     if (MINIMG) ok = ok && pi[gi].x < (TC)PAD_LIMIT && pj[gj].x < (TC)PAD_LIMIT;
     if (gi == gj) ok = ok && upper;
+    // about a fifth of the sub-tiles kept by the bounding-box test hold no pair within the
+    // cutoff: the wave skips them (uniform branch)
+    if (__builtin_amdgcn_ballot_w64(ok) == 0) continue;
     bool excl = false;
     if (EXCL) excl = (exm[gi] >> (gj * 8 + jj)) & 1u;
     // pairs that do not count are moved far away: every term below is then exactly zero
     // (exp underflows, s^6 flushes), except (erfc - 1)/r of an excluded pair beyond the
-    // cutoff, which is below 1e-12 q_i q_j
+    // cutoff, which is below 1e-12 q_i q_j (and dropped with its sub-tile when no pair of the
+    // sub-tile is in range)
     r2 = ok ? r2 : (TC)1e30;
     const TC rinv = rsqrt_t(r2);
     const TC rinv2 = rinv * rinv;
@@ -993,6 +997,7 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
         TC r2 = dx * dx + dy * dy + dz * dz;
         bool ok = r2 <= ta.rc2 && pi.x < (TC)PAD_LIMIT && pj[gj].x < (TC)PAD_LIMIT;
         if (gi == gj) ok = ok && upper;
+        if (__builtin_amdgcn_ballot_w64(ok) == 0) continue;  // no pair of the sub-tile in range
         const bool excl = (exm >> (gj * 8 + jj)) & 1u;
         r2 = ok ? r2 : (TC)1e30;
         TC f, h, phi_c, dpqi, dpqj, dfqi, dfqj, phi_lj, f_lj, dps, dfs;
