@@ -417,16 +417,28 @@ def main():
         out["mixed_vs_fused_fp64"] = {"energy_abs": abs(outs["mixed"][0] - outs["fp64"][0]),
                                       "force_rms_rel": rms_rel(outs["mixed"][1], outs["fp64"][1])}
     if head == "mixed":
+        # the reciprocal-space stages once more, alone on the caller's stream (same results)
+        from torchff import _lib as _L
+        _L.load().tff_fused_set_overlap(0)
+        try:
+            alone = kernel_times(f.step, max(5, args.steps // 5), ["fused_spread", "fused_fft", "fused_gather"])
+        finally:
+            _L.load().tff_fused_set_overlap(1)
         out["kernels_ms"] = {k: v["avg_ms"] for k, v in kern.items()}
         out["roofline"] = roof
         sb = pme_stage_bytes(w)
         out["pme_stages"] = {
             "note": "reciprocal space runs on a second stream concurrently with the pair search and pair "
-                    "kernel: stage times are wall times under that sharing (bandwidths are lower bounds)",
+                    "kernel: stage times are wall times under that sharing (bandwidths are lower bounds); "
+                    "'alone': the same stages run on the caller's stream, one after another",
             **{st: {"bytes": b, "ms": kern[f"fused_{st}"]["avg_ms"],
                     "GB/s": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9,
                     "frac_of_hbm_peak": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
-               for st, b in sb.items()}}
+               for st, b in sb.items()},
+            "alone": {st: {"ms": alone[f"fused_{st}"]["avg_ms"],
+                           "GB/s": b / (alone[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9,
+                           "frac_of_hbm_peak": b / (alone[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                      for st, b in sb.items()}}
     if head == "mixed" and not args.no_serving:
         out["serving"] = serving_results(args, dev)
     if not args.no_cpu_baseline:

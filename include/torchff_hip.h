@@ -43,6 +43,11 @@ int tff_abi_version(void);
  * "fused_spread", "fused_fft", "fused_gather". */
 void tff_prof_enable(int on);
 int tff_prof_collect(const char* name, double* total_ms, long long* count);
+/* Timing aid (no counterpart in the reference): 0 runs the reciprocal-space chain of
+ * tff_fused_energy_forces on the caller's stream after the cluster build instead of on the
+ * auxiliary stream concurrently with the pair search and pair kernel, so that its stages can be
+ * timed alone.  Results are identical either way.  Default 1. */
+void tff_fused_set_overlap(int on);
 
 /* --------------------------------------------------------- neighbor list */
 #define TFF_NBLIST_NSQUARED 0

@@ -23,6 +23,7 @@
 // the order of summation); the reciprocal-space energy is a float64 k-space sum.
 // TC = float is the mixed-precision mode, TC = double the all-float64 mode.
 
+#include <atomic>
 #include <mutex>
 
 #include "pair_math.h"
@@ -1714,22 +1715,31 @@ struct AuxStream {
 std::mutex g_aux_mu;
 AuxStream g_aux[64];
 
-hipStream_t aux_stream_fork(hipStream_t st) {
+// 0: the reciprocal-space chain runs on the caller's stream, in order (a timing aid)
+std::atomic<int> g_overlap{1};
+
+// *sp: the auxiliary stream, waiting for the work enqueued so far on st (st itself when the
+// overlap is off); false when the stream or its events could not be set up
+bool aux_stream_fork(hipStream_t st, hipStream_t* sp) {
+  *sp = st;
+  if (!g_overlap.load()) return true;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
   std::lock_guard<std::mutex> lk(g_aux_mu);
   AuxStream& a = g_aux[dev];
   if (!a.s) {
-    if (hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking) != hipSuccess) return false;
+    if (hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess) return false;
+    if (hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) return false;
   }
-  if (hipEventRecord(a.fork, st) != hipSuccess) return nullptr;
-  if (hipStreamWaitEvent(a.s, a.fork, 0) != hipSuccess) return nullptr;
-  return a.s;
+  if (hipEventRecord(a.fork, st) != hipSuccess) return false;
+  if (hipStreamWaitEvent(a.s, a.fork, 0) != hipSuccess) return false;
+  *sp = a.s;
+  return true;
 }
 
-void aux_stream_join(hipStream_t st) {
+void aux_stream_join(hipStream_t st, hipStream_t sp) {
+  if (sp == st) return;
   int dev = 0;
   hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(g_aux_mu);
@@ -1831,8 +1841,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
                      w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
   hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
                      col, w.excl_part, status);
-  hipStream_t sp = aux_stream_fork(st);
-  if (!sp) {
+  hipStream_t sp = st;
+  if (!aux_stream_fork(st, &sp)) {
     tff_set_error("fused: could not create the auxiliary stream");
     return TFF_ERR_LAUNCH;
   }
@@ -1889,7 +1899,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     tff_prof_end(tok, st);
   }
   // 6. outputs, after the reciprocal-space work
-  aux_stream_join(st);
+  aux_stream_join(st, sp);
   hipLaunchKernelGGL(k_finalize<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, w.facc_pme, forces, w.eacc,
                      w.kparts, w.nk, w.qparts, w.nq, coul, alpha, energies);
   TFF_CHECK_LAUNCH("fused energy+forces");
@@ -1999,6 +2009,8 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
 }
 
 }  // namespace
+
+extern "C" void tff_fused_set_overlap(int on) { g_overlap.store(on ? 1 : 0); }
 
 extern "C" size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const double* box_host,
                                             double list_cutoff, int K0, int K1, int K2) {
