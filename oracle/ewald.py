@@ -27,8 +27,11 @@ def half_space_m(kmax):
     return m[keep]
 
 
-def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096):
-    """Reciprocal + self energy (Gaussian units), float64 torch, differentiable in coords and q."""
+def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096, p=None):
+    """Reciprocal + self energy (Gaussian units), float64 torch, differentiable in coords, q and p.
+
+    With point dipoles p (rank 1, torchff/ewald.py:75-92): L = q + i 2 pi k.p, S = sum_j L_j
+    exp(2 pi i k.r_j), and the dipole self term (2 alpha^3 / (3 sqrt(pi))) sum |p|^2."""
     inv = torch.linalg.inv(box)
     V = torch.linalg.det(box)
     m = torch.as_tensor(half_space_m(kmax), dtype=coords.dtype)
@@ -38,11 +41,20 @@ def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096):
         ksq = (k * k).sum(1)
         g = torch.exp(-math.pi ** 2 * ksq / (alpha * alpha)) / ksq
         kr = 2.0 * math.pi * (k @ coords.T)
-        sr = (torch.cos(kr) * q[None, :]).sum(1)
-        si = (torch.sin(kr) * q[None, :]).sum(1)
+        c, s = torch.cos(kr), torch.sin(kr)
+        if p is None:
+            sr = (c * q[None, :]).sum(1)
+            si = (s * q[None, :]).sum(1)
+        else:
+            li = 2.0 * math.pi * (k @ p.T)  # (M, N)
+            sr = (c * q[None, :] - s * li).sum(1)
+            si = (c * li + s * q[None, :]).sum(1)
         e = e + (g * (sr * sr + si * si)).sum()
     e = e / (math.pi * V)
-    return e - alpha / math.sqrt(math.pi) * torch.sum(q * q)
+    e = e - alpha / math.sqrt(math.pi) * torch.sum(q * q)
+    if p is not None:
+        e = e - 2.0 * alpha ** 3 / (3.0 * math.sqrt(math.pi)) * torch.sum(p * p)
+    return e
 
 
 def real_space_all_images(coords, box, q, alpha, rmax):

@@ -119,6 +119,39 @@ def pme_energy(coords, box, q, K, alpha, order, return_phi=False):
     return E
 
 
+def spread_dipoles(coords, box, q, p, K, order):
+    """Charge grid of point charges and point dipoles (pme_ref.py:145-184, rank 1):
+    Q(m) += q_i W_i(m) + p_i . grad_r W_i(m).  With u = m0 - s + const (du/ds = -1) and
+    ds_a/dr_x = K_a (B^-1)_{x a}: p . grad_r W = -sum_a pu_a dW/du_a, pu_a = K_a (p B^-1)_a
+    (the reference's p_u = p Nj_Aji_star^T, W_tot -= p_u . grad_W_u)."""
+    M, dM, idx, inv = _weights(coords, box, K, order)
+    Ks = torch.tensor(_ktuple(K), dtype=coords.dtype)
+    pu = (p @ inv) * Ks[None, :]
+    W = M[:, 0, :, None, None] * M[:, 1, None, :, None] * M[:, 2, None, None, :]
+    dW0 = dM[:, 0, :, None, None] * M[:, 1, None, :, None] * M[:, 2, None, None, :]
+    dW1 = M[:, 0, :, None, None] * dM[:, 1, None, :, None] * M[:, 2, None, None, :]
+    dW2 = M[:, 0, :, None, None] * M[:, 1, None, :, None] * dM[:, 2, None, None, :]
+    val = q[:, None, None, None] * W - (pu[:, 0, None, None, None] * dW0 + pu[:, 1, None, None, None] * dW1
+                                         + pu[:, 2, None, None, None] * dW2)
+    flat = _flat_index(idx, K)
+    K0, K1, K2 = _ktuple(K)
+    Q = torch.zeros(K0 * K1 * K2, dtype=coords.dtype)
+    Q = Q.index_add(0, flat.reshape(-1), val.reshape(-1))
+    return Q.reshape(K0, K1, K2)
+
+
+def pme_energy_dipoles(coords, box, q, p, K, alpha, order):
+    """Rank-1 PME energy (reciprocal + self) in Gaussian units: 1/2 sum_k G |FFT(Q)|^2 with Q from
+    :func:`spread_dipoles`, minus alpha/sqrt(pi) sum q^2 and (2 alpha^3 / (3 sqrt(pi))) sum |p|^2
+    (the self terms of pme_ref.py:435-453 and torchff/ewald.py:91-92).  The reference's potential,
+    field and forces are its derivatives: phi_i = dE/dq_i, E_i = -dE/dp_i, F_i = -dE/dr_i."""
+    Q = spread_dipoles(coords, box, q, p, K, order)
+    G = influence(box, K, alpha, order)
+    S = torch.fft.fftn(Q)
+    return (0.5 * torch.sum(G * (S.real ** 2 + S.imag ** 2)) - alpha / math.sqrt(math.pi) * torch.sum(q * q)
+            - 2.0 * alpha ** 3 / (3.0 * math.sqrt(math.pi)) * torch.sum(p * p))
+
+
 def pme_energy_kspace(coords, box, q, K, alpha, order):
     """Same energy from the k-space sum 1/2 sum_k G |S|^2 - alpha/sqrt(pi) sum q^2."""
     Q = spread(coords, box, q, K, order)
