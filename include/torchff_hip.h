@@ -120,6 +120,35 @@ int tff_pme_gather(int dtype, const void* phi_grid, const void* coords, const vo
                    const double* box_host, int K0, int K1, int K2, int order, double alpha, double scale,
                    void* epot_out, void* dcoords_out, tff_stream_t stream);
 
+/* Multipolar form of tff_pme_energy: replaces pme_long_range / pme_long_range_all for ranks 1
+ * and 2 (csrc/pme/pme.cu:19-168,251-414; spread_q_kernel<RANK> csrc/pme/kernels.cuh:12-187;
+ * Python torchff/pme_ref.py:145-235,425-469).  dipoles (N x 3) and quadrupoles (N x 3 x 3,
+ * row-major, taken as given: the self terms are those of traceless quadrupoles) may be NULL,
+ * and so may charges when dipoles or quadrupoles are given.  Each atom adds
+ * q W + p . grad W + (1/3) theta : grad grad W to the charge grid; energy_out = scale * (E_recip
+ * + E_self) with E_self = -a/sqrt(pi) sum (q^2 + (2a^2/3)|p|^2 + (8a^4/45) theta:theta).
+ * Quadrupoles need spline order >= 4.  Same workspace as tff_pme_energy. */
+int tff_pme_energy_multipole(int dtype, const void* coords, const void* charges, const void* dipoles,
+                             const void* quadrupoles, long long natoms, const double* box_host, int K0, int K1, int K2,
+                             int order, double alpha, const void* xmoduli, const void* ymoduli, const void* zmoduli,
+                             int zmoduli_half, double scale, void* energy_out, void* phi_grid_out, void* workspace,
+                             size_t workspace_bytes, tff_stream_t stream);
+
+/* Interpolation for multipoles (csrc/pme/kernels.cuh:191-470 interpolate_kernel<RANK>,
+ * csrc/pme/kernels_with_field.cuh interpolate_kernel_with_field), all outputs scaled by scale
+ * and each may be NULL:
+ *   epot_out (N)          = dE/dq  = phi - 2a q/sqrt(pi)
+ *   field_out (N x 3)     = -dE/dp = -grad phi + 4a^3/(3 sqrt(pi)) p
+ *   field_grad_out (N x 9)= -3 dE/dtheta = -grad grad phi + 16a^5/(15 sqrt(pi)) theta
+ *   dcoords_out (N x 3)   = dE/dr  = q grad phi + (grad grad phi) p + (1/3) theta : grad^3 phi
+ * for the potential grid phi_grid of tff_pme_energy(_multipole) and the multipoles given here
+ * (which may differ from those that made the grid: the backward of pme_long_range_all contracts
+ * one grid with two multipole sets). */
+int tff_pme_gather_multipole(int dtype, const void* phi_grid, const void* coords, const void* charges,
+                             const void* dipoles, const void* quadrupoles, long long natoms, const double* box_host,
+                             int K0, int K1, int K2, int order, double alpha, double scale, void* epot_out,
+                             void* field_out, void* field_grad_out, void* dcoords_out, tff_stream_t stream);
+
 /* ------------------------------------------------------- direct Ewald */
 size_t tff_ewald_workspace_bytes(int dtype, long long natoms, int kmax);
 

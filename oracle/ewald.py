@@ -27,11 +27,13 @@ def half_space_m(kmax):
     return m[keep]
 
 
-def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096, p=None):
-    """Reciprocal + self energy (Gaussian units), float64 torch, differentiable in coords, q and p.
+def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096, p=None, th=None):
+    """Reciprocal + self energy (Gaussian units), float64 torch, differentiable in coords, q, p, th.
 
-    With point dipoles p (rank 1, torchff/ewald.py:75-92): L = q + i 2 pi k.p, S = sum_j L_j
-    exp(2 pi i k.r_j), and the dipole self term (2 alpha^3 / (3 sqrt(pi))) sum |p|^2."""
+    With point dipoles p and Cartesian quadrupoles th (N, 3, 3) (ranks 1 and 2,
+    torchff/ewald.py:75-97): L = q + i 2 pi k.p - (4 pi^2/3) k.th.k, S = sum_j L_j
+    exp(2 pi i k.r_j), and the self terms (2 alpha^3 / (3 sqrt(pi))) sum |p|^2 and
+    (8 alpha^5 / (45 sqrt(pi))) sum th:th (traceless quadrupoles)."""
     inv = torch.linalg.inv(box)
     V = torch.linalg.det(box)
     m = torch.as_tensor(half_space_m(kmax), dtype=coords.dtype)
@@ -42,7 +44,13 @@ def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096, p=None):
         g = torch.exp(-math.pi ** 2 * ksq / (alpha * alpha)) / ksq
         kr = 2.0 * math.pi * (k @ coords.T)
         c, s = torch.cos(kr), torch.sin(kr)
-        if p is None:
+        if th is not None:
+            kk = torch.einsum("mx,nxy,my->mn", k, th, k)
+            qe = q[None, :] - (4.0 * math.pi ** 2 / 3.0) * kk
+            li = 2.0 * math.pi * (k @ p.T) if p is not None else torch.zeros_like(qe)
+            sr = (c * qe - s * li).sum(1)
+            si = (c * li + s * qe).sum(1)
+        elif p is None:
             sr = (c * q[None, :]).sum(1)
             si = (s * q[None, :]).sum(1)
         else:
@@ -54,6 +62,8 @@ def ewald_recip_energy(coords, box, q, kmax, alpha, chunk=4096, p=None):
     e = e - alpha / math.sqrt(math.pi) * torch.sum(q * q)
     if p is not None:
         e = e - 2.0 * alpha ** 3 / (3.0 * math.sqrt(math.pi)) * torch.sum(p * p)
+    if th is not None:
+        e = e - 8.0 * alpha ** 5 / (45.0 * math.sqrt(math.pi)) * torch.sum(th * th)
     return e
 
 

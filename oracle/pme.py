@@ -1,7 +1,8 @@
 """Smooth-PME reciprocal-space oracle, any spline order, float64 torch (test infrastructure only).
 
 Restates torchff/pme_ref.py for rank 0 (point charges), generalised from the
-hard-wired order 6 to order p:
+hard-wired order 6 to order p (ranks 1 and 2, point dipoles and quadrupoles, in
+:func:`spread_multipoles` / :func:`pme_energy_multipoles`):
 
 * scaled fractional coordinates s = K (r . B^-1) (pme_ref.py:11-17,24);
 * anchor and spline arguments from :func:`oracle.bspline.anchor`
@@ -25,6 +26,11 @@ Pinned by tests/test_oracle_pme.py against the FFT-free Ewald sum of
 :mod:`oracle.ewald` (relative error falls with K and p), the NaCl Madelung
 constant, order-6 moduli against the reference's cosine form, the energy
 identity 1/2 sum q phi = 1/2 sum_k G |S|^2, and finite-difference forces.
+The multipolar energy is pinned by tests/test_oracle_dipoles.py against the
+direct Ewald sum with L = q + 2 pi i k.p - (4 pi^2/3) k.theta.k, against the
+limit of a charge pair (dipole) and of a four-charge cluster (quadrupole) with
+their intra-cluster Gaussian terms removed in closed form, and by finite
+differences.
 """
 
 import math
@@ -119,20 +125,51 @@ def pme_energy(coords, box, q, K, alpha, order, return_phi=False):
     return E
 
 
-def spread_dipoles(coords, box, q, p, K, order):
-    """Charge grid of point charges and point dipoles (pme_ref.py:145-184, rank 1):
-    Q(m) += q_i W_i(m) + p_i . grad_r W_i(m).  With u = m0 - s + const (du/ds = -1) and
-    ds_a/dr_x = K_a (B^-1)_{x a}: p . grad_r W = -sum_a pu_a dW/du_a, pu_a = K_a (p B^-1)_a
-    (the reference's p_u = p Nj_Aji_star^T, W_tot -= p_u . grad_W_u)."""
+def _lattice_R(box, K, dtype):
+    """R_ax = K_a (B^-1)_{xa}: ds_a/dr_x, the reference's Nj_Aji_star (pme_ref.py:15-20)."""
+    Ks = torch.tensor(_ktuple(K), dtype=dtype)
+    return Ks[:, None] * torch.linalg.inv(box).T
+
+
+def spread_multipoles(coords, box, q, p, th, K, order):
+    """Charge grid of point charges, dipoles and Cartesian quadrupoles (pme_ref.py:145-235):
+    Q(m) += q W(m) + p . grad_r W(m) + (1/3) theta : grad_r grad_r W(m).
+    With u = m0 - s + const (du/ds = -1) and ds_a/dr_x = R_ax:
+    p . grad_r W = -sum_a pu_a dW/du_a with pu = R p (the reference's W_tot -= p_u . grad_W_u), and
+    theta : grad grad W = sum_ab (R theta R^T)_ab d2W/du_a du_b (its Theta_u contraction, scale 1/3).
+    p (N, 3) and th (N, 3, 3) may be None; q may be zeros."""
     M, dM, idx, inv = _weights(coords, box, K, order)
-    Ks = torch.tensor(_ktuple(K), dtype=coords.dtype)
-    pu = (p @ inv) * Ks[None, :]
     W = M[:, 0, :, None, None] * M[:, 1, None, :, None] * M[:, 2, None, None, :]
-    dW0 = dM[:, 0, :, None, None] * M[:, 1, None, :, None] * M[:, 2, None, None, :]
-    dW1 = M[:, 0, :, None, None] * dM[:, 1, None, :, None] * M[:, 2, None, None, :]
-    dW2 = M[:, 0, :, None, None] * M[:, 1, None, :, None] * dM[:, 2, None, None, :]
-    val = q[:, None, None, None] * W - (pu[:, 0, None, None, None] * dW0 + pu[:, 1, None, None, None] * dW1
-                                         + pu[:, 2, None, None, None] * dW2)
+    val = q[:, None, None, None] * W
+    R = _lattice_R(box, K, coords.dtype)
+    if p is not None:
+        pu = p @ R.T
+        dW0 = dM[:, 0, :, None, None] * M[:, 1, None, :, None] * M[:, 2, None, None, :]
+        dW1 = M[:, 0, :, None, None] * dM[:, 1, None, :, None] * M[:, 2, None, None, :]
+        dW2 = M[:, 0, :, None, None] * M[:, 1, None, :, None] * dM[:, 2, None, None, :]
+        val = val - (pu[:, 0, None, None, None] * dW0 + pu[:, 1, None, None, None] * dW1
+                     + pu[:, 2, None, None, None] * dW2)
+    if th is not None:
+        Ks = torch.tensor(_ktuple(K), dtype=coords.dtype)
+        s = (coords @ inv) * Ks
+        _, u0 = bspline.anchor(s, order)
+        sh = torch.as_tensor(bspline.stencil(order), dtype=coords.dtype)
+        d2M = bspline.bspline_d2M(u0[:, :, None] + sh[None, None, :], order)
+        U = torch.einsum("ax,nxy,by->nab", R, th, R)
+        X = [M[:, a] for a in range(3)]
+        D = [dM[:, a] for a in range(3)]
+        D2 = [d2M[:, a] for a in range(3)]
+
+        def outer(f0, f1, f2):
+            return f0[:, :, None, None] * f1[:, None, :, None] * f2[:, None, None, :]
+
+        H = {(0, 0): outer(D2[0], X[1], X[2]), (1, 1): outer(X[0], D2[1], X[2]), (2, 2): outer(X[0], X[1], D2[2]),
+             (0, 1): outer(D[0], D[1], X[2]), (0, 2): outer(D[0], X[1], D[2]), (1, 2): outer(X[0], D[1], D[2])}
+        quad = 0.0
+        for (a, b_), h in H.items():
+            c = U[:, a, b_] if a == b_ else U[:, a, b_] + U[:, b_, a]
+            quad = quad + c[:, None, None, None] * h
+        val = val + quad / 3.0
     flat = _flat_index(idx, K)
     K0, K1, K2 = _ktuple(K)
     Q = torch.zeros(K0 * K1 * K2, dtype=coords.dtype)
@@ -140,16 +177,22 @@ def spread_dipoles(coords, box, q, p, K, order):
     return Q.reshape(K0, K1, K2)
 
 
-def pme_energy_dipoles(coords, box, q, p, K, alpha, order):
-    """Rank-1 PME energy (reciprocal + self) in Gaussian units: 1/2 sum_k G |FFT(Q)|^2 with Q from
-    :func:`spread_dipoles`, minus alpha/sqrt(pi) sum q^2 and (2 alpha^3 / (3 sqrt(pi))) sum |p|^2
-    (the self terms of pme_ref.py:435-453 and torchff/ewald.py:91-92).  The reference's potential,
-    field and forces are its derivatives: phi_i = dE/dq_i, E_i = -dE/dp_i, F_i = -dE/dr_i."""
-    Q = spread_dipoles(coords, box, q, p, K, order)
+def pme_energy_multipoles(coords, box, q, p, th, K, alpha, order):
+    """Multipolar PME energy (reciprocal + self) in Gaussian units: 1/2 sum_k G |FFT(Q)|^2 with Q
+    from :func:`spread_multipoles`, minus alpha/sqrt(pi) sum (q^2 + (2 alpha^2/3) |p|^2
+    + (8 alpha^4/45) theta : theta) (csrc/ewald/self_contribution.cuh:4-110; pme_ref.py:435-469;
+    the quadrupole term is the self energy of traceless quadrupoles).  The reference's potential,
+    field and field gradient are its derivatives: phi_i = dE/dq_i, E_i = -dE/dp_i,
+    grad E_i = -3 dE/dtheta_i, F_i = -dE/dr_i."""
+    Q = spread_multipoles(coords, box, q, p, th, K, order)
     G = influence(box, K, alpha, order)
     S = torch.fft.fftn(Q)
-    return (0.5 * torch.sum(G * (S.real ** 2 + S.imag ** 2)) - alpha / math.sqrt(math.pi) * torch.sum(q * q)
-            - 2.0 * alpha ** 3 / (3.0 * math.sqrt(math.pi)) * torch.sum(p * p))
+    e = 0.5 * torch.sum(G * (S.real ** 2 + S.imag ** 2)) - alpha / math.sqrt(math.pi) * torch.sum(q * q)
+    if p is not None:
+        e = e - 2.0 * alpha ** 3 / (3.0 * math.sqrt(math.pi)) * torch.sum(p * p)
+    if th is not None:
+        e = e - 8.0 * alpha ** 5 / (45.0 * math.sqrt(math.pi)) * torch.sum(th * th)
+    return e
 
 
 def pme_energy_kspace(coords, box, q, K, alpha, order):

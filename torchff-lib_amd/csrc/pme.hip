@@ -1,6 +1,8 @@
-// Smooth-PME reciprocal space for point charges on gfx950, float32 or float64.
+// Smooth-PME reciprocal space for point charges, dipoles and quadrupoles on gfx950, float32 or
+// float64.
 //
-// Replaces the reference's pme_long_range op for rank 0
+// Replaces the reference's pme_long_range / pme_long_range_all ops (ranks 0..2; the multipole
+// spread and interpolation are described at k_spread / k_gather below) for rank 0
 // (csrc/pme/pme.cu:15-248: spread_q_kernel, torch rfftn, pme_convolution_fused_kernel,
 // irfftn, 1/2 sum(Phi Q), self term; backward interpolate_kernel), generalised from
 // the hard-wired spline order 6 to orders 3..8.
@@ -33,101 +35,342 @@ namespace {
 
 using namespace tffpme;
 
-template <typename T, int P>
-__global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, const T* __restrict__ q, long long n, PmeGeom g,
-                         double* __restrict__ grid) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double r0 = (double)coords[3 * i], r1 = (double)coords[3 * i + 1], r2 = (double)coords[3 * i + 2];
-  T Mv[3][P], dM[3][P];
-  int base[3];
+// Per-atom multipoles.  Level 0: charges only; level 1: charges and dipoles; level 2: also
+// Cartesian quadrupoles theta (3 x 3 row-major per atom).  An atom contributes to the charge grid
+//   Q(m) += q W(m) + p . grad_r W(m) + (1/3) theta : grad_r grad_r W(m)
+// (pme_ref.py:145-235).  With u = m0 - s + const, d/ds_a = -d/du_a and ds_a/dr_x = R_ax =
+// K_a (B^-1)_{xa}, so p . grad_r W = -sum_a pu_a dW/du_a with pu = R p, and
+// theta : grad grad W = sum_ab (R theta R^T)_ab d2W/du_a du_b.
+struct Multipoles {
+  const void* q;   // N, may be NULL (zero) at levels >= 1
+  const void* p;   // N x 3, NULL = zero
+  const void* th;  // N x 9, NULL = zero
+};
+
+template <typename T>
+__device__ __forceinline__ double mp_load(const void* a, long long k) {
+  return (double)((const T*)a)[k];
+}
+
+// lattice-frame dipole pu_a = sum_x R_ax p_x and quadrupole coefficients tc = (1/3) R theta R^T
+// symmetrised: tc[0..2] diagonal aa, tc[3] = ab01, tc[4] = 02, tc[5] = 12 (off-diagonals doubled,
+// as each appears twice in the contraction)
+template <typename T>
+__device__ __forceinline__ void lattice_multipoles(const Multipoles& mp, long long i, const PmeGeom& g, double pu[3],
+                                                   double tc[6]) {
+  double R[3][3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    double s = (r0 * g.inv[a] + r1 * g.inv[3 + a] + r2 * g.inv[6 + a]) * (double)g.K[a];
-    double t;
-    anchor_axis(s, g.K[a], P, base[a], t);
-    spline_weights<T, P>((T)t, Mv[a], dM[a]);
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) R[a][x] = (double)g.K[a] * g.inv[3 * x + a];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) pu[a] = 0.0;
+  if (mp.p) {
+    const double p0 = mp_load<T>(mp.p, 3 * i), p1 = mp_load<T>(mp.p, 3 * i + 1), p2 = mp_load<T>(mp.p, 3 * i + 2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pu[a] = R[a][0] * p0 + R[a][1] * p1 + R[a][2] * p2;
   }
-  const T qi = q[i];
-  const int K1 = g.K[1], K2 = g.K[2];
-  int gx = base[0];
-  for (int a = 0; a < P; ++a) {
-    int gy = base[1];
-    const T wa = qi * Mv[0][a];
-    for (int b = 0; b < P; ++b) {
-      const T wab = wa * Mv[1][b];
-      double* row = grid + ((long long)gx * K1 + gy) * K2;
-      int gz = base[2];
 #pragma unroll
-      for (int c = 0; c < P; ++c) {
-        atomicAdd(&row[gz], (double)(wab * Mv[2][c]));
-        gz = gz + 1 == K2 ? 0 : gz + 1;
-      }
-      gy = gy + 1 == K1 ? 0 : gy + 1;
-    }
-    gx = gx + 1 == g.K[0] ? 0 : gx + 1;
+  for (int k = 0; k < 6; ++k) tc[k] = 0.0;
+  if (mp.th) {
+    double th[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) th[k] = mp_load<T>(mp.th, 9 * i + k);
+    double tR[3][3];  // theta R^T: tR[x][b] = sum_y th[x][y] R[b][y]
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) tR[x][b] = th[3 * x] * R[b][0] + th[3 * x + 1] * R[b][1] + th[3 * x + 2] * R[b][2];
+    double U[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) U[a][b] = R[a][0] * tR[0][b] + R[a][1] * tR[1][b] + R[a][2] * tR[2][b];
+    const double third = 1.0 / 3.0;
+    tc[0] = U[0][0] * third;
+    tc[1] = U[1][1] * third;
+    tc[2] = U[2][2] * third;
+    tc[3] = (U[0][1] + U[1][0]) * third;
+    tc[4] = (U[0][2] + U[2][0]) * third;
+    tc[5] = (U[1][2] + U[2][1]) * third;
   }
 }
 
-template <typename T, int P>
-__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ phi_grid, const T* __restrict__ coords, const T* __restrict__ q,
-                         long long n, PmeGeom g, double scale, T* __restrict__ epot, T* __restrict__ dcoords) {
+template <typename T, int P, int LVL>
+__global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Multipoles mp, long long n, PmeGeom g,
+                                                double* __restrict__ grid) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double r0 = (double)coords[3 * i], r1 = (double)coords[3 * i + 1], r2 = (double)coords[3 * i + 2];
-  T Mv[3][P], dM[3][P];
+  constexpr int PD = LVL >= 2 ? P : 1;
+  T Mv[3][P], dM[3][P], d2M[3][PD];
   int base[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     double s = (r0 * g.inv[a] + r1 * g.inv[3 + a] + r2 * g.inv[6 + a]) * (double)g.K[a];
     double t;
     anchor_axis(s, g.K[a], P, base[a], t);
-    spline_weights<T, P>((T)t, Mv[a], dM[a]);
+    if constexpr (LVL >= 2)
+      spline_weights2<T, P>((T)t, Mv[a], dM[a], d2M[a]);
+    else
+      spline_weights<T, P>((T)t, Mv[a], dM[a]);
+  }
+  const int K1 = g.K[1], K2 = g.K[2];
+  if constexpr (LVL == 0) {
+    const T qi = ((const T*)mp.q)[i];
+    int gx = base[0];
+    for (int a = 0; a < P; ++a) {
+      int gy = base[1];
+      const T wa = qi * Mv[0][a];
+      for (int b = 0; b < P; ++b) {
+        const T wab = wa * Mv[1][b];
+        double* row = grid + ((long long)gx * K1 + gy) * K2;
+        int gz = base[2];
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          atomicAdd(&row[gz], (double)(wab * Mv[2][c]));
+          gz = gz + 1 == K2 ? 0 : gz + 1;
+        }
+        gy = gy + 1 == K1 ? 0 : gy + 1;
+      }
+      gx = gx + 1 == g.K[0] ? 0 : gx + 1;
+    }
+  } else {
+    double pud[3], tcd[6];
+    lattice_multipoles<T>(mp, i, g, pud, tcd);
+    const T qi = mp.q ? ((const T*)mp.q)[i] : (T)0;
+    const T pu0 = (T)pud[0], pu1 = (T)pud[1], pu2 = (T)pud[2];
+    T tc[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tc[k] = (T)tcd[k];
+    int gx = base[0];
+    for (int a = 0; a < P; ++a) {
+      int gy = base[1];
+      for (int b = 0; b < P; ++b) {
+        const T xy = Mv[0][a] * Mv[1][b], dxy = dM[0][a] * Mv[1][b], xdy = Mv[0][a] * dM[1][b];
+        // value at z-stencil point c: c0 M(c) + c1 M'(c) + c2 M''(c)
+        T c0 = qi * xy - pu0 * dxy - pu1 * xdy;
+        T c1 = -pu2 * xy;
+        T c2 = (T)0;
+        if constexpr (LVL >= 2) {
+          c0 += tc[0] * d2M[0][a] * Mv[1][b] + tc[1] * Mv[0][a] * d2M[1][b] + tc[3] * dM[0][a] * dM[1][b];
+          c1 += tc[4] * dxy + tc[5] * xdy;
+          c2 = tc[2] * xy;
+        }
+        double* row = grid + ((long long)gx * K1 + gy) * K2;
+        int gz = base[2];
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          T v = c0 * Mv[2][c] + c1 * dM[2][c];
+          if constexpr (LVL >= 2) v += c2 * d2M[2][c];
+          atomicAdd(&row[gz], (double)v);
+          gz = gz + 1 == K2 ? 0 : gz + 1;
+        }
+        gy = gy + 1 == K1 ? 0 : gy + 1;
+      }
+      gx = gx + 1 == g.K[0] ? 0 : gx + 1;
+    }
+  }
+}
+
+// Interpolation of the potential grid at each atom.  With phi_i = sum_m W_i(m) Phi(m) and its
+// lattice derivatives g_a, h_ab, c_abc (u-frame), the Cartesian derivatives are
+// grad phi = -R^T g, grad grad phi = R^T h R, grad grad grad phi = -R^T R^T R^T c, and
+//   dE/dq = phi - 2 a q/sqrt(pi)                                  (potential)
+//   -dE/dp = -grad phi + 4 a^3/(3 sqrt(pi)) p                     (field)
+//   -3 dE/dtheta = -grad grad phi + 16 a^5/(15 sqrt(pi)) theta    (field gradient)
+//   dE/dr = q grad phi + (grad grad phi) p + (1/3) theta : grad grad grad phi
+// (pme_ref.py:334-469 conventions; csrc/pme/kernels.cuh:191-470 for the charges-only case).
+// Levels: 0 = first derivatives, 1 = + Hessian, 2 = + third derivatives.
+template <typename T, int P, int LVL>
+__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ phi_grid, const T* __restrict__ coords,
+                                                Multipoles mp, long long n, PmeGeom g, double scale,
+                                                T* __restrict__ epot, T* __restrict__ field, T* __restrict__ fgrad,
+                                                T* __restrict__ dcoords) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r0 = (double)coords[3 * i], r1 = (double)coords[3 * i + 1], r2 = (double)coords[3 * i + 2];
+  constexpr int P2 = LVL >= 1 ? P : 1, P3 = LVL >= 2 ? P : 1;
+  T Mv[3][P], dM[3][P], d2M[3][P2], d3M[3][P3];
+  int base[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double s = (r0 * g.inv[a] + r1 * g.inv[3 + a] + r2 * g.inv[6 + a]) * (double)g.K[a];
+    double t;
+    anchor_axis(s, g.K[a], P, base[a], t);
+    if constexpr (LVL >= 2)
+      spline_weights3<T, P>((T)t, Mv[a], dM[a], d2M[a], d3M[a]);
+    else if constexpr (LVL == 1)
+      spline_weights2<T, P>((T)t, Mv[a], dM[a], d2M[a]);
+    else
+      spline_weights<T, P>((T)t, Mv[a], dM[a]);
   }
   const int K1 = g.K[1], K2 = g.K[2];
   T phi = 0, du0 = 0, du1 = 0, du2 = 0;
+  T h00 = 0, h11 = 0, h22 = 0, h01 = 0, h02 = 0, h12 = 0;
+  T c000 = 0, c111 = 0, c222 = 0, c001 = 0, c002 = 0, c011 = 0, c022 = 0, c112 = 0, c122 = 0, c012 = 0;
   int gx = base[0];
   for (int a = 0; a < P; ++a) {
     int gy = base[1];
     for (int b = 0; b < P; ++b) {
       const T* row = phi_grid + ((long long)gx * K1 + gy) * K2;
       int gz = base[2];
-      T s0 = 0, s1 = 0;
+      T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
       for (int c = 0; c < P; ++c) {
         T v = row[gz];
         s0 += v * Mv[2][c];
         s1 += v * dM[2][c];
+        if constexpr (LVL >= 1) s2 += v * d2M[2][c];
+        if constexpr (LVL >= 2) s3 += v * d3M[2][c];
         gz = gz + 1 == K2 ? 0 : gz + 1;
       }
       phi += Mv[0][a] * Mv[1][b] * s0;
       du0 += dM[0][a] * Mv[1][b] * s0;
       du1 += Mv[0][a] * dM[1][b] * s0;
       du2 += Mv[0][a] * Mv[1][b] * s1;
+      if constexpr (LVL >= 1) {
+        h00 += d2M[0][a] * Mv[1][b] * s0;
+        h11 += Mv[0][a] * d2M[1][b] * s0;
+        h22 += Mv[0][a] * Mv[1][b] * s2;
+        h01 += dM[0][a] * dM[1][b] * s0;
+        h02 += dM[0][a] * Mv[1][b] * s1;
+        h12 += Mv[0][a] * dM[1][b] * s1;
+      }
+      if constexpr (LVL >= 2) {
+        c000 += d3M[0][a] * Mv[1][b] * s0;
+        c111 += Mv[0][a] * d3M[1][b] * s0;
+        c222 += Mv[0][a] * Mv[1][b] * s3;
+        c001 += d2M[0][a] * dM[1][b] * s0;
+        c002 += d2M[0][a] * Mv[1][b] * s1;
+        c011 += dM[0][a] * d2M[1][b] * s0;
+        c022 += dM[0][a] * Mv[1][b] * s2;
+        c112 += Mv[0][a] * d2M[1][b] * s1;
+        c122 += Mv[0][a] * dM[1][b] * s2;
+        c012 += dM[0][a] * dM[1][b] * s1;
+      }
       gy = gy + 1 == K1 ? 0 : gy + 1;
     }
     gx = gx + 1 == g.K[0] ? 0 : gx + 1;
   }
-  // weights depend on t = m0 - s + const, so d/ds = -d/dt; ds_a/dr_j = K_a inv[j][a]
-  const double qi = (double)q[i];
-  const double d0 = -(double)du0 * g.K[0], d1 = -(double)du1 * g.K[1], d2 = -(double)du2 * g.K[2];
-  const double gxr = d0 * g.inv[0] + d1 * g.inv[1] + d2 * g.inv[2];
-  const double gyr = d0 * g.inv[3] + d1 * g.inv[4] + d2 * g.inv[5];
-  const double gzr = d0 * g.inv[6] + d1 * g.inv[7] + d2 * g.inv[8];
-  const double self = -2.0 * g.alpha * 0.5641895835477563 * qi;  // 1/sqrt(pi)
-  if (epot) epot[i] = (T)(scale * ((double)phi + self));
-  if (dcoords) {
-    dcoords[3 * i] = (T)(scale * qi * gxr);
-    dcoords[3 * i + 1] = (T)(scale * qi * gyr);
-    dcoords[3 * i + 2] = (T)(scale * qi * gzr);
+  constexpr double RPI = 0.5641895835477563;  // 1/sqrt(pi)
+  const double al = g.alpha;
+  if constexpr (LVL == 0) {
+    // weights depend on t = m0 - s + const, so d/ds = -d/dt; ds_a/dr_j = K_a inv[j][a]
+    const double qi = (double)((const T*)mp.q)[i];
+    const double d0 = -(double)du0 * g.K[0], d1 = -(double)du1 * g.K[1], d2 = -(double)du2 * g.K[2];
+    const double gxr = d0 * g.inv[0] + d1 * g.inv[1] + d2 * g.inv[2];
+    const double gyr = d0 * g.inv[3] + d1 * g.inv[4] + d2 * g.inv[5];
+    const double gzr = d0 * g.inv[6] + d1 * g.inv[7] + d2 * g.inv[8];
+    const double self = -2.0 * al * RPI * qi;
+    if (epot) epot[i] = (T)(scale * ((double)phi + self));
+    if (field) {
+      field[3 * i] = (T)(-scale * gxr);
+      field[3 * i + 1] = (T)(-scale * gyr);
+      field[3 * i + 2] = (T)(-scale * gzr);
+    }
+    if (dcoords) {
+      dcoords[3 * i] = (T)(scale * qi * gxr);
+      dcoords[3 * i + 1] = (T)(scale * qi * gyr);
+      dcoords[3 * i + 2] = (T)(scale * qi * gzr);
+    }
+  } else {
+    double R[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) R[a][x] = (double)g.K[a] * g.inv[3 * x + a];
+    const double qi = mp.q ? mp_load<T>(mp.q, i) : 0.0;
+    double pu[3], tc[6];
+    lattice_multipoles<T>(mp, i, g, pu, tc);
+    const double gu[3] = {(double)du0, (double)du1, (double)du2};
+    const double H[3][3] = {{(double)h00, (double)h01, (double)h02},
+                            {(double)h01, (double)h11, (double)h12},
+                            {(double)h02, (double)h12, (double)h22}};
+    // lattice-frame vector v with dE/dr = R^T v:
+    //   v_a = -q g_a + sum_b h_ab pu_b - sum_bc c_abc (1/3)(R theta R^T)_bc
+    double v[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] = -qi * gu[a] + H[a][0] * pu[0] + H[a][1] * pu[1] + H[a][2] * pu[2];
+    if constexpr (LVL >= 2) {
+      // tc holds (1/3) U with off-diagonals doubled, so sum_bc c_abc U_bc / 3 =
+      // c_a00 tc0 + c_a11 tc1 + c_a22 tc2 + c_a01 tc3 + c_a02 tc4 + c_a12 tc5
+      const double C0[6] = {(double)c000, (double)c011, (double)c022, (double)c001, (double)c002, (double)c012};
+      const double C1[6] = {(double)c001, (double)c111, (double)c122, (double)c011, (double)c012, (double)c112};
+      const double C2[6] = {(double)c002, (double)c112, (double)c222, (double)c012, (double)c022, (double)c122};
+      double s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        s0 += C0[k] * tc[k];
+        s1 += C1[k] * tc[k];
+        s2 += C2[k] * tc[k];
+      }
+      v[0] -= s0;
+      v[1] -= s1;
+      v[2] -= s2;
+    }
+    double grad[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) grad[x] = -(R[0][x] * gu[0] + R[1][x] * gu[1] + R[2][x] * gu[2]);
+    if (epot) epot[i] = (T)(scale * ((double)phi - 2.0 * al * RPI * qi));
+    if (field) {
+      const double cf = 4.0 * al * al * al * RPI / 3.0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) field[3 * i + x] = (T)(scale * (-grad[x] + (mp.p ? cf * mp_load<T>(mp.p, 3 * i + x) : 0.0)));
+    }
+    if (fgrad) {
+      // grad grad phi = R^T H R
+      const double cg = 16.0 * al * al * al * al * al * RPI / 15.0;
+      double HR[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int y = 0; y < 3; ++y) HR[a][y] = H[a][0] * R[0][y] + H[a][1] * R[1][y] + H[a][2] * R[2][y];
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+          const double hxy = R[0][x] * HR[0][y] + R[1][x] * HR[1][y] + R[2][x] * HR[2][y];
+          fgrad[9 * i + 3 * x + y] =
+              (T)(scale * (-hxy + (mp.th ? cg * mp_load<T>(mp.th, 9 * i + 3 * x + y) : 0.0)));
+        }
+    }
+    if (dcoords) {
+#pragma unroll
+      for (int x = 0; x < 3; ++x) dcoords[3 * i + x] = (T)(scale * (R[0][x] * v[0] + R[1][x] * v[1] + R[2][x] * v[2]));
+    }
   }
 }
 
-__global__ void k_self_energy_parts(const void* q, int dtype, long long n, double alpha, double* part) {
+// self energy -a/sqrt(pi) sum (q^2 + (2 a^2/3) |p|^2 + (8 a^4/45) theta : theta)
+// (csrc/ewald/self_contribution.cuh:4-110)
+template <typename T>
+__global__ void k_self_energy_parts(Multipoles mp, long long n, double alpha, double* part) {
   double acc = 0.0;
+  const double a2 = alpha * alpha;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double qi = dtype == TFF_F64 ? ((const double*)q)[i] : (double)((const float*)q)[i];
-    acc += qi * qi;
+    double e = 0.0;
+    if (mp.q) {
+      const double qi = mp_load<T>(mp.q, i);
+      e = qi * qi;
+    }
+    if (mp.p) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double v = mp_load<T>(mp.p, 3 * i + k);
+        s += v * v;
+      }
+      e += 2.0 * a2 / 3.0 * s;
+    }
+    if (mp.th) {
+      double s = 0.0;
+      for (int k = 0; k < 9; ++k) {
+        const double v = mp_load<T>(mp.th, 9 * i + k);
+        s += v * v;
+      }
+      e += 8.0 * a2 * a2 / 45.0 * s;
+    }
+    acc += e;
   }
   double tot = tff_block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = -alpha * 0.5641895835477563 * tot;
@@ -176,44 +419,74 @@ PmeWs carve(void* base, int dtype, const int K[3]) {
   return w;
 }
 
+// kernel level of a multipole set: 0 = charges only, 1 = dipoles (or charges absent), 2 = quadrupoles
+int spread_level(const Multipoles& mp) { return mp.th ? 2 : (mp.p || !mp.q) ? 1 : 0; }
+
+#define TFF_ORDER_DISPATCH(CALL)                                                            \
+  switch (order) {                                                                          \
+    case 3: CALL(3); break;                                                                 \
+    case 4: CALL(4); break;                                                                 \
+    case 5: CALL(5); break;                                                                 \
+    case 6: CALL(6); break;                                                                 \
+    case 7: CALL(7); break;                                                                 \
+    case 8: CALL(8); break;                                                                 \
+    default: tff_set_error("PME: unsupported spline order %d (3..8)", order); return TFF_ERR_UNSUPPORTED; \
+  }
+
+template <typename T, int L>
+int spread_level_launch(int order, unsigned nb, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g,
+                        double* grid, hipStream_t st) {
+#define TFF_S(P) hipLaunchKernelGGL((k_spread<T, P, L>), dim3(nb), dim3(256), 0, st, coords, mp, n, g, grid)
+  TFF_ORDER_DISPATCH(TFF_S)
+#undef TFF_S
+  return TFF_OK;
+}
+
 template <typename T>
-int launch_spread(int order, const T* coords, const T* q, long long n, const PmeGeom& g, double* grid,
+int launch_spread(int order, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g, double* grid,
                   hipStream_t st) {
   unsigned nb = (unsigned)((n + 255) / 256);
   if (n == 0) return TFF_OK;
   void* tok = tff_prof_begin("pme_spread", st);
-  switch (order) {
-    case 3: hipLaunchKernelGGL((k_spread<T, 3>), dim3(nb), dim3(256), 0, st, coords, q, n, g, grid); break;
-    case 4: hipLaunchKernelGGL((k_spread<T, 4>), dim3(nb), dim3(256), 0, st, coords, q, n, g, grid); break;
-    case 5: hipLaunchKernelGGL((k_spread<T, 5>), dim3(nb), dim3(256), 0, st, coords, q, n, g, grid); break;
-    case 6: hipLaunchKernelGGL((k_spread<T, 6>), dim3(nb), dim3(256), 0, st, coords, q, n, g, grid); break;
-    case 7: hipLaunchKernelGGL((k_spread<T, 7>), dim3(nb), dim3(256), 0, st, coords, q, n, g, grid); break;
-    case 8: hipLaunchKernelGGL((k_spread<T, 8>), dim3(nb), dim3(256), 0, st, coords, q, n, g, grid); break;
-    default: tff_set_error("PME: unsupported spline order %d (3..8)", order); return TFF_ERR_UNSUPPORTED;
-  }
+  const int lvl = spread_level(mp);
+  int rc = lvl == 0   ? spread_level_launch<T, 0>(order, nb, coords, mp, n, g, grid, st)
+           : lvl == 1 ? spread_level_launch<T, 1>(order, nb, coords, mp, n, g, grid, st)
+                      : spread_level_launch<T, 2>(order, nb, coords, mp, n, g, grid, st);
+  if (rc) return rc;
   tff_prof_end(tok, st);
   TFF_CHECK_LAUNCH("k_spread");
   return TFF_OK;
 }
 
+template <typename T, int P, int L>
+void gather_one(unsigned nb, const T* phi, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g,
+                double scale, T* epot, T* field, T* fgrad, T* dcoords, hipStream_t st) {
+  if constexpr (L < 2 || P >= 4)  // third derivatives need order >= 4 (checked by the callers)
+    hipLaunchKernelGGL((k_gather<T, P, L>), dim3(nb), dim3(256), 0, st, phi, coords, mp, n, g, scale, epot, field,
+                       fgrad, dcoords);
+}
+
+template <typename T, int L>
+int gather_level_launch(int order, unsigned nb, const T* phi, const T* coords, const Multipoles& mp, long long n,
+                        const PmeGeom& g, double scale, T* epot, T* field, T* fgrad, T* dcoords, hipStream_t st) {
+#define TFF_G(P) gather_one<T, P, L>(nb, phi, coords, mp, n, g, scale, epot, field, fgrad, dcoords, st)
+  TFF_ORDER_DISPATCH(TFF_G)
+#undef TFF_G
+  return TFF_OK;
+}
+
 template <typename T>
-int launch_gather(int order, const T* phi, const T* coords, const T* q, long long n, const PmeGeom& g, double scale,
-                  T* epot, T* dcoords, hipStream_t st) {
+int launch_gather(int order, const T* phi, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g,
+                  double scale, T* epot, T* field, T* fgrad, T* dcoords, hipStream_t st) {
   unsigned nb = (unsigned)((n + 255) / 256);
   if (n == 0) return TFF_OK;
   void* tok = tff_prof_begin("pme_gather", st);
-#define TFF_G(P) \
-  hipLaunchKernelGGL((k_gather<T, P>), dim3(nb), dim3(256), 0, st, phi, coords, q, n, g, scale, epot, dcoords)
-  switch (order) {
-    case 3: TFF_G(3); break;
-    case 4: TFF_G(4); break;
-    case 5: TFF_G(5); break;
-    case 6: TFF_G(6); break;
-    case 7: TFF_G(7); break;
-    case 8: TFF_G(8); break;
-    default: tff_set_error("PME: unsupported spline order %d (3..8)", order); return TFF_ERR_UNSUPPORTED;
-  }
-#undef TFF_G
+  const int lvl = mp.th ? 2 : (mp.p || !mp.q || fgrad) ? 1 : 0;
+  int rc = lvl == 0 ? gather_level_launch<T, 0>(order, nb, phi, coords, mp, n, g, scale, epot, field, fgrad, dcoords, st)
+           : lvl == 1
+               ? gather_level_launch<T, 1>(order, nb, phi, coords, mp, n, g, scale, epot, field, fgrad, dcoords, st)
+               : gather_level_launch<T, 2>(order, nb, phi, coords, mp, n, g, scale, epot, field, fgrad, dcoords, st);
+  if (rc) return rc;
   tff_prof_end(tok, st);
   TFF_CHECK_LAUNCH("k_gather");
   return TFF_OK;
@@ -231,7 +504,7 @@ int launch_pass(const PassCfg& c, const FftPlan& plan, const ConvArgs& cv, const
 }
 
 template <typename T>
-int pme_energy(const T* coords, const T* q, long long n, const TffBox& box, const int K[3], int order, double alpha,
+int pme_energy(const T* coords, const Multipoles& mp, long long n, const TffBox& box, const int K[3], int order, double alpha,
                const T* mx, const T* my, const T* mz, int mz_half, double scale, T* energy, T* phi_out,
                const PmeWs& w, hipStream_t st) {
   FftPlan pl[3];
@@ -247,7 +520,7 @@ int pme_energy(const T* coords, const T* q, long long n, const TffBox& box, cons
     if (b > 8192) b = 8192;
     hipLaunchKernelGGL(k_zero_grid, dim3((unsigned)b), dim3(256), 0, st, w.grid, ng);
   }
-  int rc = launch_spread<T>(order, coords, q, n, g, w.grid, st);
+  int rc = launch_spread<T>(order, coords, mp, n, g, w.grid, st);
   if (rc) return rc;
   const size_t ce = sizeof(cx<T>);
   ConvArgs cv{};
@@ -267,8 +540,8 @@ int pme_energy(const T* coords, const T* q, long long n, const TffBox& box, cons
   if ((rc = launch_pass<T, PASS_X_CONV>(cxp, pl[0], cv, w.work, w.work, mx, my, mz, mz_half, w.parts, st))) return rc;
   if ((rc = launch_pass<T, PASS_Y_INV>(cy, pl[1], cv, w.work, w.work, mx, my, mz, mz_half, w.parts, st))) return rc;
   if ((rc = launch_pass<T, PASS_Z_INV>(cz, pl[2], cv, w.work, phi_out, mx, my, mz, mz_half, w.parts, st))) return rc;
-  hipLaunchKernelGGL(k_self_energy_parts, dim3(SELF_BLOCKS), dim3(256), 0, st, (const void*)q,
-                     (int)(sizeof(T) == 8 ? TFF_F64 : TFF_F32), n, alpha, w.parts + cxp.tiles);
+  hipLaunchKernelGGL(k_self_energy_parts<T>, dim3(SELF_BLOCKS), dim3(256), 0, st, mp, n, alpha,
+                     w.parts + cxp.tiles);
   if (energy)
     hipLaunchKernelGGL(k_pme_energy_final<T>, dim3(1), dim3(256), 0, st, w.parts, cxp.tiles + SELF_BLOCKS, scale,
                        energy);
@@ -307,55 +580,116 @@ extern "C" size_t tff_pme_workspace_bytes(int dtype, long long natoms, int K0, i
   return carve(nullptr, dtype, K).bytes;
 }
 
-extern "C" int tff_pme_energy(int dtype, const void* coords, const void* charges, long long natoms,
-                              const double* box_host, int K0, int K1, int K2, int order, double alpha,
-                              const void* xmoduli, const void* ymoduli, const void* zmoduli, int zmoduli_half,
-                              double scale, void* energy_out, void* phi_grid_out, void* workspace,
-                              size_t workspace_bytes, hipStream_t stream) {
+namespace {
+
+int pme_energy_entry(const char* name, bool need_q, int dtype, const void* coords, const Multipoles& mp,
+                     long long natoms,
+                     const double* box_host, int K0, int K1, int K2, int order, double alpha, const void* xmoduli,
+                     const void* ymoduli, const void* zmoduli, int zmoduli_half, double scale, void* energy_out,
+                     void* phi_grid_out, void* workspace, size_t workspace_bytes, hipStream_t stream) {
   int K[3] = {K0, K1, K2};
   int rc = check_geom(box_host, K, order);
   if (rc) return rc;
-  if (natoms < 0 || (natoms > 0 && (!coords || !charges)) || !xmoduli || !ymoduli || !zmoduli || !phi_grid_out) {
-    tff_set_error("tff_pme_energy: invalid arguments");
+  if (natoms < 0 || (natoms > 0 && (!coords || (need_q && !mp.q))) || !xmoduli || !ymoduli || !zmoduli ||
+      !phi_grid_out) {
+    tff_set_error("%s: invalid arguments", name);
     return TFF_ERR_ARG;
+  }
+  if (mp.th && order < 4) {
+    tff_set_error("%s: quadrupoles need spline order >= 4 (got %d)", name, order);
+    return TFF_ERR_UNSUPPORTED;
   }
   size_t need = tff_pme_workspace_bytes(dtype, natoms, K0, K1, K2);
   if (workspace_bytes < need || !workspace) {
-    tff_set_error("tff_pme_energy: workspace too small (%zu < %zu)", workspace_bytes, need);
+    tff_set_error("%s: workspace too small (%zu < %zu)", name, workspace_bytes, need);
     return TFF_ERR_WORKSPACE;
   }
   TffBox box = tff_make_box(box_host);
   PmeWs w = carve(workspace, dtype, K);
   if (dtype == TFF_F64)
-    return pme_energy<double>((const double*)coords, (const double*)charges, natoms, box, K, order, alpha,
-                              (const double*)xmoduli, (const double*)ymoduli, (const double*)zmoduli, zmoduli_half,
-                              scale, (double*)energy_out, (double*)phi_grid_out, w, stream);
+    return pme_energy<double>((const double*)coords, mp, natoms, box, K, order, alpha, (const double*)xmoduli,
+                              (const double*)ymoduli, (const double*)zmoduli, zmoduli_half, scale,
+                              (double*)energy_out, (double*)phi_grid_out, w, stream);
   if (dtype == TFF_F32)
-    return pme_energy<float>((const float*)coords, (const float*)charges, natoms, box, K, order, alpha,
-                             (const float*)xmoduli, (const float*)ymoduli, (const float*)zmoduli, zmoduli_half,
-                             scale, (float*)energy_out, (float*)phi_grid_out, w, stream);
-  tff_set_error("tff_pme_energy: unsupported dtype %d", dtype);
+    return pme_energy<float>((const float*)coords, mp, natoms, box, K, order, alpha, (const float*)xmoduli,
+                             (const float*)ymoduli, (const float*)zmoduli, zmoduli_half, scale, (float*)energy_out,
+                             (float*)phi_grid_out, w, stream);
+  tff_set_error("%s: unsupported dtype %d", name, dtype);
   return TFF_ERR_ARG;
+}
+
+int pme_gather_entry(const char* name, bool need_q, int dtype, const void* phi_grid, const void* coords,
+                     const Multipoles& mp,
+                     long long natoms, const double* box_host, int K0, int K1, int K2, int order, double alpha,
+                     double scale, void* epot_out, void* field_out, void* fgrad_out, void* dcoords_out,
+                     hipStream_t stream) {
+  int K[3] = {K0, K1, K2};
+  int rc = check_geom(box_host, K, order);
+  if (rc) return rc;
+  if (natoms < 0 || (natoms > 0 && (!coords || !phi_grid || (need_q && !mp.q)))) {
+    tff_set_error("%s: invalid arguments", name);
+    return TFF_ERR_ARG;
+  }
+  if (mp.th && order < 4) {
+    tff_set_error("%s: quadrupoles need spline order >= 4 (got %d)", name, order);
+    return TFF_ERR_UNSUPPORTED;
+  }
+  TffBox box = tff_make_box(box_host);
+  PmeGeom g = make_geom(box, K, order, alpha);
+  if (dtype == TFF_F64)
+    return launch_gather<double>(order, (const double*)phi_grid, (const double*)coords, mp, natoms, g, scale,
+                                 (double*)epot_out, (double*)field_out, (double*)fgrad_out, (double*)dcoords_out,
+                                 stream);
+  if (dtype == TFF_F32)
+    return launch_gather<float>(order, (const float*)phi_grid, (const float*)coords, mp, natoms, g, scale,
+                                (float*)epot_out, (float*)field_out, (float*)fgrad_out, (float*)dcoords_out, stream);
+  tff_set_error("%s: unsupported dtype %d", name, dtype);
+  return TFF_ERR_ARG;
+}
+
+}  // namespace
+
+extern "C" int tff_pme_energy(int dtype, const void* coords, const void* charges, long long natoms,
+                              const double* box_host, int K0, int K1, int K2, int order, double alpha,
+                              const void* xmoduli, const void* ymoduli, const void* zmoduli, int zmoduli_half,
+                              double scale, void* energy_out, void* phi_grid_out, void* workspace,
+                              size_t workspace_bytes, hipStream_t stream) {
+  const Multipoles mp{charges, nullptr, nullptr};
+  return pme_energy_entry("tff_pme_energy", true, dtype, coords, mp, natoms, box_host, K0, K1, K2, order, alpha,
+                          xmoduli,
+                          ymoduli, zmoduli, zmoduli_half, scale, energy_out, phi_grid_out, workspace, workspace_bytes,
+                          stream);
 }
 
 extern "C" int tff_pme_gather(int dtype, const void* phi_grid, const void* coords, const void* charges,
                               long long natoms, const double* box_host, int K0, int K1, int K2, int order,
                               double alpha, double scale, void* epot_out, void* dcoords_out, hipStream_t stream) {
-  int K[3] = {K0, K1, K2};
-  int rc = check_geom(box_host, K, order);
-  if (rc) return rc;
-  if (natoms < 0 || (natoms > 0 && (!coords || !charges || !phi_grid))) {
-    tff_set_error("tff_pme_gather: invalid arguments");
-    return TFF_ERR_ARG;
-  }
-  TffBox box = tff_make_box(box_host);
-  PmeGeom g = make_geom(box, K, order, alpha);
-  if (dtype == TFF_F64)
-    return launch_gather<double>(order, (const double*)phi_grid, (const double*)coords, (const double*)charges, natoms,
-                                 g, scale, (double*)epot_out, (double*)dcoords_out, stream);
-  if (dtype == TFF_F32)
-    return launch_gather<float>(order, (const float*)phi_grid, (const float*)coords, (const float*)charges, natoms, g,
-                                scale, (float*)epot_out, (float*)dcoords_out, stream);
-  tff_set_error("tff_pme_gather: unsupported dtype %d", dtype);
-  return TFF_ERR_ARG;
+  const Multipoles mp{charges, nullptr, nullptr};
+  return pme_gather_entry("tff_pme_gather", true, dtype, phi_grid, coords, mp, natoms, box_host, K0, K1, K2, order,
+                          alpha,
+                          scale, epot_out, nullptr, nullptr, dcoords_out, stream);
+}
+
+extern "C" int tff_pme_energy_multipole(int dtype, const void* coords, const void* charges, const void* dipoles,
+                                        const void* quadrupoles, long long natoms, const double* box_host, int K0,
+                                        int K1, int K2, int order, double alpha, const void* xmoduli,
+                                        const void* ymoduli, const void* zmoduli, int zmoduli_half, double scale,
+                                        void* energy_out, void* phi_grid_out, void* workspace,
+                                        size_t workspace_bytes, hipStream_t stream) {
+  const Multipoles mp{charges, dipoles, quadrupoles};
+  return pme_energy_entry("tff_pme_energy_multipole", !dipoles && !quadrupoles, dtype, coords, mp, natoms, box_host,
+                          K0, K1, K2, order, alpha,
+                          xmoduli, ymoduli, zmoduli, zmoduli_half, scale, energy_out, phi_grid_out, workspace,
+                          workspace_bytes, stream);
+}
+
+extern "C" int tff_pme_gather_multipole(int dtype, const void* phi_grid, const void* coords, const void* charges,
+                                        const void* dipoles, const void* quadrupoles, long long natoms,
+                                        const double* box_host, int K0, int K1, int K2, int order, double alpha,
+                                        double scale, void* epot_out, void* field_out, void* field_grad_out,
+                                        void* dcoords_out, hipStream_t stream) {
+  const Multipoles mp{charges, dipoles, quadrupoles};
+  return pme_gather_entry("tff_pme_gather_multipole", !dipoles && !quadrupoles, dtype, phi_grid, coords, mp, natoms,
+                          box_host, K0, K1, K2,
+                          order, alpha, scale, epot_out, field_out, field_grad_out, dcoords_out, stream);
 }

@@ -100,6 +100,44 @@ __device__ __forceinline__ void spline_weights2(T t, T* v, T* dv, T* d2v) {
   for (int k = 0; k < P; ++k) v[k] = w[k];
 }
 
+// as spline_weights2, plus d3v[k] = third derivative of M_P at t + k = M_{P-3}(t+k)
+// - 3 M_{P-3}(t+k-1) + 3 M_{P-3}(t+k-2) - M_{P-3}(t+k-3); needs P >= 4
+template <typename T, int P>
+__device__ __forceinline__ void spline_weights3(T t, T* v, T* dv, T* d2v, T* d3v) {
+  static_assert(P >= 4, "third derivative needs spline order >= 4");
+  T w[P];
+  w[0] = (T)1;
+#pragma unroll
+  for (int k = 1; k < P; ++k) w[k] = (T)0;
+#pragma unroll
+  for (int n = 2; n <= P; ++n) {
+    if (n == P - 2) {
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        d3v[k] = w[k] - (T)3 * (k > 0 ? w[k - 1] : (T)0) + (T)3 * (k > 1 ? w[k - 2] : (T)0) -
+                 (k > 2 ? w[k - 3] : (T)0);
+    }
+    if (n == P - 1) {
+#pragma unroll
+      for (int k = 0; k < P; ++k)
+        d2v[k] = w[k] - (T)2 * (k > 0 ? w[k - 1] : (T)0) + (k > 1 ? w[k - 2] : (T)0);
+    }
+    if (n == P) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) dv[k] = w[k] - (k > 0 ? w[k - 1] : (T)0);
+    }
+    const T div = (T)1 / (T)(n - 1);
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      T a = (k < n - 1) ? w[k] : (T)0;
+      T b = (k > 0) ? w[k - 1] : (T)0;
+      w[k] = ((t + (T)k) * a + ((T)n - t - (T)k) * b) * div;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) v[k] = w[k];
+}
+
 // grid base (already wrapped into [0, K)) and spline parameter t for one axis
 __device__ __forceinline__ void anchor_axis(double s, int K, int P, int& base, double& t) {
   double shift = (P & 1) ? 0.5 : 0.0;

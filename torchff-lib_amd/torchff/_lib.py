@@ -39,6 +39,10 @@ _SIGNATURES = {
     "tff_pme_energy": (_i, [_i, _vp, _vp, _ll, _vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _i, _d, _vp, _vp, _vp,
                             _sz, _vp]),
     "tff_pme_gather": (_i, [_i, _vp, _vp, _vp, _ll, _vp, _i, _i, _i, _i, _d, _d, _vp, _vp, _vp]),
+    "tff_pme_energy_multipole": (_i, [_i, _vp, _vp, _vp, _vp, _ll, _vp, _i, _i, _i, _i, _d, _vp, _vp, _vp, _i, _d,
+                                      _vp, _vp, _vp, _sz, _vp]),
+    "tff_pme_gather_multipole": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _ll, _vp, _i, _i, _i, _i, _d, _d, _vp, _vp, _vp,
+                                      _vp, _vp]),
     "tff_ewald_workspace_bytes": (_sz, [_i, _ll, _i]),
     "tff_ewald": (_i, [_i, _vp, _vp, _ll, _vp, _i, _d, _vp, _vp, _vp, _vp, _sz, _vp]),
     "tff_fused_workspace_bytes": (_sz, [_i, _ll, _vp, _d, _i, _i, _i]),

@@ -221,18 +221,25 @@ def _grid_dims(K):
     return (int(K),) * 3
 
 
+def _pme_setup(coords, box, K, xm, ym, zm):
+    L.require_cuda(coords, box, xm, ym, zm)
+    dt = coords.dtype
+    K0, K1, K2 = _grid_dims(K)
+    xm_, ym_, zm_ = (m.detach().to(dt).contiguous() for m in (xm, ym, zm))
+    if xm_.numel() != K0 or ym_.numel() != K1 or zm_.numel() not in (K2, K2 // 2 + 1):
+        raise RuntimeError("pme_long_range: moduli lengths must be (K, K, K//2+1) or (K, K, K)")
+    zhalf = int(zm_.numel() != K2 or K2 == 1)
+    return (K0, K1, K2), (xm_, ym_, zm_, zhalf)
+
+
 def _pme_forward(coords, box, q, K, alpha, xm, ym, zm, order):
     """Energy and the potential grid; returns (energy, coords, charges, phi, (K0, K1, K2, order, alpha))."""
     L.require_cuda(coords, box, q, xm, ym, zm)
     lib = L.load()
     dt = coords.dtype
-    K0, K1, K2 = _grid_dims(K)
+    (K0, K1, K2), (xm_, ym_, zm_, zhalf) = _pme_setup(coords, box, K, xm, ym, zm)
     c = coords.detach().contiguous()
     qq = q.detach().to(dt).contiguous()
-    xm_, ym_, zm_ = (m.detach().to(dt).contiguous() for m in (xm, ym, zm))
-    if xm_.numel() != K0 or ym_.numel() != K1 or zm_.numel() not in (K2, K2 // 2 + 1):
-        raise RuntimeError("pme_long_range: moduli lengths must be (K, K, K//2+1) or (K, K, K)")
-    zhalf = int(zm_.numel() != K2 or K2 == 1)
     bh = L.box_host(box)
     n = c.shape[0]
     energy = torch.empty((), dtype=dt, device=c.device)
@@ -272,9 +279,126 @@ class _PmeEnergy(torch.autograd.Function):
             None, None
 
 
+# ---- multipoles (ranks 1 and 2)
+
+def _as_quadrupole9(t, n):
+    """(N, 9) row-major Cartesian quadrupoles from (N, 3, 3), (N, 9) or the packed (N, 6)
+    [xx, xy, xz, yy, yz, zz] of torchff/pme.py:30-64 (differentiable)."""
+    if t.ndim == 3 and t.shape[1:] == (3, 3):
+        return t.reshape(n, 9)
+    if t.ndim == 2 and t.shape[1] == 9:
+        return t
+    if t.ndim == 2 and t.shape[1] == 6:
+        xx, xy, xz, yy, yz, zz = t.unbind(1)
+        return torch.stack([xx, xy, xz, xy, yy, yz, xz, yz, zz], dim=1)
+    raise ValueError(f"pme_long_range: quadrupole tensor t has unexpected shape {tuple(t.shape)}")
+
+
+def _mp_energy(c, qq, pp, tt, box, dims, mod, order, alpha):
+    """Spread + FFT + influence for one multipole set; returns (energy, phi grid)."""
+    lib = L.load()
+    K0, K1, K2 = dims
+    xm_, ym_, zm_, zhalf = mod
+    n = c.shape[0]
+    energy = torch.empty((), dtype=c.dtype, device=c.device)
+    phi = torch.empty((K0, K1, K2), dtype=c.dtype, device=c.device)
+    ws = L.workspace(lib.tff_pme_workspace_bytes(L.dtype_code(c), n, K0, K1, K2), c.device)
+    rc = lib.tff_pme_energy_multipole(L.dtype_code(c), L.ptr(c), L.ptr(qq), L.ptr(pp), L.ptr(tt), n,
+                                      L.box_host(box), K0, K1, K2, int(order), float(alpha), L.ptr(xm_), L.ptr(ym_),
+                                      L.ptr(zm_), zhalf, 1.0, L.ptr(energy), L.ptr(phi), L.ptr(ws), ws.numel(),
+                                      L.stream_of(c))
+    L.check(rc, "pme_long_range")
+    return energy, phi
+
+
+def _mp_gather(phi, c, qq, pp, tt, box, dims, order, alpha, pot=False, field=False, fgrad=False, dcoords=False):
+    lib = L.load()
+    n = c.shape[0]
+    o_pot = torch.empty(n, dtype=c.dtype, device=c.device) if pot else None
+    o_field = torch.empty(n, 3, dtype=c.dtype, device=c.device) if field else None
+    o_fg = torch.empty(n, 9, dtype=c.dtype, device=c.device) if fgrad else None
+    o_dc = torch.empty(n, 3, dtype=c.dtype, device=c.device) if dcoords else None
+    K0, K1, K2 = dims
+    rc = lib.tff_pme_gather_multipole(L.dtype_code(c), L.ptr(phi), L.ptr(c), L.ptr(qq), L.ptr(pp), L.ptr(tt), n,
+                                      L.box_host(box), K0, K1, K2, int(order), float(alpha), 1.0, L.ptr(o_pot),
+                                      L.ptr(o_field), L.ptr(o_fg), L.ptr(o_dc), L.stream_of(c))
+    L.check(rc, "pme_long_range gather")
+    return o_pot, o_field, o_fg, o_dc
+
+
+def _mp_prepare(coords, q, p, t9):
+    dt = coords.dtype
+    c = coords.detach().contiguous()
+    qq = q.detach().to(dt).contiguous()
+    pp = None if p is None else p.detach().to(dt).reshape(-1, 3).contiguous()
+    tt = None if t9 is None else t9.detach().to(dt).contiguous()
+    return c, qq, pp, tt
+
+
+class _PmeMultipole(torch.autograd.Function):
+    """Multipolar PME (csrc/pme/pme.cu:19-248 and 251-576 for ranks 1, 2).  Returns (energy,
+    potential, field, field gradient): potential = dE/dq, field = -dE/dp, field gradient
+    = -3 dE/dtheta (all with the self terms).  The backward handles upstream gradients on the
+    energy, the potential and the field, as the reference's PMEAllFunction does: with
+    L = (q, p, theta) and M = (g_pot, -g_field, 0), the loss gE E + g_pot.pot + g_field.field has
+    d/d(q, p, theta) = the (potential, -field, -field gradient/3) of L' = gE L + M, and
+    d/dr = [L contracted with the grid of L'] + [M contracted with the grid of L]."""
+
+    @staticmethod
+    def forward(ctx, coords, box, q, p, t9, K, alpha, xm, ym, zm, order):
+        dims, mod = _pme_setup(coords, box, K, xm, ym, zm)
+        c, qq, pp, tt = _mp_prepare(coords, q, p, t9)
+        energy, phi = _mp_energy(c, qq, pp, tt, box, dims, mod, order, alpha)
+        pot, field, fg, dc = _mp_gather(phi, c, qq, pp, tt, box, dims, order, alpha, pot=True, field=True,
+                                        fgrad=tt is not None, dcoords=True)
+        if fg is None:
+            fg = torch.zeros(c.shape[0], 9, dtype=c.dtype, device=c.device)
+        ctx.save_for_backward(c, qq, pp, tt, phi, pot, field, fg, dc)
+        ctx.box, ctx.dims, ctx.mod, ctx.order, ctx.alpha = box, dims, mod, order, alpha
+        ctx.q_dtype = q.dtype
+        ctx.mark_non_differentiable(fg)
+        ctx.set_materialize_grads(False)  # unused outputs give None: energy-only backward stays one gather
+        return energy, pot, field, fg
+
+    @staticmethod
+    def backward(ctx, ge, gpot, gfield, _gfg):
+        c, qq, pp, tt, phi, pot, field, fg, dc = ctx.saved_tensors
+        has_p, has_t = pp is not None, tt is not None
+        n = c.shape[0]
+        if ge is None:
+            ge = torch.zeros((), dtype=c.dtype, device=c.device)
+        s = ge.reshape(()).to(c.dtype)
+        if gpot is None and gfield is None:
+            dcoords, dq, dp, dt9 = dc * s, pot * s, -field * s, -fg * (s / 3.0)
+        else:
+            mq = gpot.to(c.dtype) if gpot is not None else torch.zeros_like(qq)
+            mp_ = -gfield.to(c.dtype) if gfield is not None else torch.zeros_like(c)
+            q2 = (qq * s + mq).contiguous()
+            p2 = ((pp if has_p else torch.zeros_like(c)) * s + mp_).contiguous()
+            t2 = (tt * s).contiguous() if has_t else None
+            _, phi2 = _mp_energy(c, q2, p2, t2, ctx.box, ctx.dims, ctx.mod, ctx.order, ctx.alpha)
+            pot2, field2, fg2, _ = _mp_gather(phi2, c, q2, p2, t2, ctx.box, ctx.dims, ctx.order, ctx.alpha,
+                                              pot=True, field=True, fgrad=has_t)
+            _, _, _, dc1 = _mp_gather(phi2, c, qq, pp, tt, ctx.box, ctx.dims, ctx.order, ctx.alpha, dcoords=True)
+            _, _, _, dc2 = _mp_gather(phi, c, mq.contiguous(), mp_.contiguous(), None, ctx.box, ctx.dims, ctx.order,
+                                      ctx.alpha, dcoords=True)
+            dcoords, dq, dp = dc1 + dc2, pot2, -field2
+            dt9 = -fg2 / 3.0 if has_t else None
+        return (dcoords, None, dq.to(ctx.q_dtype), (dp if has_p else None), (dt9 if has_t else None), None, None,
+                None, None, None, None)
+
+
+def _pme_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order):
+    n = coords.shape[0]
+    t9 = None if t is None else _as_quadrupole9(t, n)
+    if t9 is not None and int(order) < 4:
+        raise RuntimeError("pme_long_range: quadrupoles need spline order >= 4")
+    return _PmeMultipole.apply(coords, box, q, p, t9, int(K), float(alpha), xmoduli, ymoduli, zmoduli, int(order))
+
+
 def _pme_long_range(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order=6):
     if p is not None or t is not None:
-        raise NotImplementedError("pme_long_range: only rank 0 (point charges) is provided by the gfx950 build")
+        return _pme_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order)[0]
     return _PmeEnergy.apply(coords, box, q, int(K), float(alpha), xmoduli, ymoduli, zmoduli, int(order))
 
 
@@ -285,10 +409,12 @@ _def.define("pme_long_range_all(Tensor coords, Tensor box, Tensor q, Tensor? p, 
 
 
 class _PmeAll(torch.autograd.Function):
-    """(energy, potential, field) for point charges (csrc/pme/pme.cu:251-414): the potential is
+    """(energy, potential, field) for point charges (csrc/pme/pme.cu:251-576): the potential is
     dE/dq including the self term, from the gather of the same forward call; the field of a
     rank-0 evaluation is zero, as the reference returns it (its interpolation writes the field
-    for ranks >= 1 only, csrc/pme/kernels.cuh:407-411); gradients flow from the energy."""
+    for ranks >= 1 only, csrc/pme/kernels.cuh:407-411).  Upstream gradients on the energy and the
+    potential both flow back (the reference's backward spreads q gE + g_pot): the charges get the
+    potential of q gE + g_pot, the coordinates g_pot grad phi + q grad phi'."""
 
     @staticmethod
     def forward(ctx, coords, box, q, K, alpha, xm, ym, zm, order):
@@ -301,23 +427,36 @@ class _PmeAll(torch.autograd.Function):
         rc = lib.tff_pme_gather(L.dtype_code(c), L.ptr(phi), L.ptr(c), L.ptr(qq), n, L.box_host(box), K0, K1, K2,
                                 order_, alpha_, 1.0, L.ptr(pot), L.ptr(dc), L.stream_of(c))
         L.check(rc, "pme_long_range_all")
-        ctx.save_for_backward(pot, dc)
+        ctx.save_for_backward(c, qq, phi, pot, dc)
+        ctx.box, ctx.K, ctx.alpha, ctx.mods, ctx.order = box, K, alpha, (xm, ym, zm), order
+        ctx.q_dtype = q.dtype
         field = torch.zeros_like(c)
-        ctx.mark_non_differentiable(pot, field)
+        ctx.mark_non_differentiable(field)
+        ctx.set_materialize_grads(False)
         return e, pot, field
 
     @staticmethod
     def backward(ctx, ge, gp, gf):
-        pot, dc = ctx.saved_tensors
+        c, qq, phi, pot, dc = ctx.saved_tensors
         if ge is None:
-            return (None,) * 9
-        s = ge.reshape(())
-        return dc * s, None, pot * s, None, None, None, None, None, None
+            ge = torch.zeros((), dtype=c.dtype, device=c.device)
+        s = ge.reshape(()).to(c.dtype)
+        if gp is None:
+            return dc * s, None, (pot * s).to(ctx.q_dtype), None, None, None, None, None, None
+        dims, mod = _pme_setup(c, ctx.box, ctx.K, *ctx.mods)
+        mq = gp.to(c.dtype).contiguous()
+        q2 = (qq * s + mq).contiguous()
+        _, phi2 = _mp_energy(c, q2, None, None, ctx.box, dims, mod, ctx.order, ctx.alpha)
+        pot2 = _mp_gather(phi2, c, q2, None, None, ctx.box, dims, ctx.order, ctx.alpha, pot=True)[0]
+        _, _, _, dc1 = _mp_gather(phi2, c, qq, None, None, ctx.box, dims, ctx.order, ctx.alpha, dcoords=True)
+        _, _, _, dc2 = _mp_gather(phi, c, mq, None, None, ctx.box, dims, ctx.order, ctx.alpha, dcoords=True)
+        return dc1 + dc2, None, pot2.to(ctx.q_dtype), None, None, None, None, None, None
 
 
 def _pme_long_range_all(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order=6):
     if p is not None or t is not None:
-        raise NotImplementedError("pme_long_range_all: only rank 0 (point charges) is provided by the gfx950 build")
+        e, pot, field, _ = _pme_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order)
+        return e, pot, field
     return _PmeAll.apply(coords, box, q, int(K), float(alpha), xmoduli, ymoduli, zmoduli, int(order))
 
 

@@ -1,12 +1,14 @@
-"""Smooth PME for point charges (drop-in for torchff/pme.py of the reference, rank 0).
+"""Smooth PME for point charges, dipoles and quadrupoles (drop-in for torchff/pme.py).
 
 ``PME(alpha, max_hkl, rank)`` keeps the reference's constructor and forward
 (torchff/pme.py:13-127) and adds ``order`` (B-spline order, default 6 as in
-the reference; 5 for the 96k- and 999k-atom workloads).  ``forward`` returns
-the reciprocal-space plus self energy in Gaussian units (multiply by the
-Coulomb constant), differentiable in coords and q; with ``return_fields=True``
-it returns (energy, per-atom potential incl. the self term, field), the field
-being zero for point charges as in the reference (``pme_long_range_all``).
+the reference; 5 for the 96k- and 999k-atom workloads).  ``forward(coords,
+box, q, p=None, t=None)`` returns the reciprocal-space plus self energy in
+Gaussian units (multiply by the Coulomb constant), differentiable in coords, q,
+p and t; with ``return_fields=True`` it returns (energy, per-atom potential,
+field), both including the self terms, the field being zero for point charges
+as in the reference (``pme_long_range_all``).  Quadrupoles t may be (N, 3, 3),
+(N, 9) or the packed (N, 6) [xx, xy, xz, yy, yz, zz].
 """
 
 from typing import Optional
@@ -23,8 +25,8 @@ class PME(nn.Module):
     def __init__(self, alpha: float, max_hkl: int, rank: int, use_customized_ops: bool = True,
                  return_fields: bool = False, order: int = 6):
         super().__init__()
-        if rank != 0:
-            raise NotImplementedError("PME: only rank 0 (point charges) is provided by the gfx950 build")
+        if rank not in (0, 1, 2):
+            raise ValueError(f"PME: rank must be 0, 1 or 2 (got {rank})")
         require_custom(use_customized_ops, "PME")
         self.alpha = float(alpha)
         self.max_hkl = int(max_hkl)
