@@ -164,6 +164,20 @@ int tff_ewald(int dtype, const void* coords, const void* charges, long long nato
               double alpha, double* energy_out, void* potential_out, void* forces_out, void* workspace,
               size_t workspace_bytes, tff_stream_t stream);
 
+/* Multipolar form of tff_ewald (ranks 1, 2: csrc/ewald/ewald.cu:9-610, torchff/ewald.py:60-110):
+ * L(k) = q - (4 pi^2/3) k.theta.k + i 2 pi k.p, S = sum_j L_j exp(2 pi i k.r_j).  Source multipoles
+ * charges (N), dipoles (N x 3), quadrupoles (N x 3 x 3 row-major), each may be NULL (zero).
+ * energy_out (device double) = 1/(pi V) sum G |S|^2 - a/sqrt(pi) sum (q^2 + (2a^2/3)|p|^2
+ * + (8a^4/45) theta:theta); potential_out = dE/dq, field_out = -dE/dp, field_grad_out (N x 9) =
+ * -3 dE/dtheta, all with the self terms; forces_out (N x 3) = -dE/dr, or, when any of c_charges /
+ * c_dipoles / c_quadrupoles is given, the force of the set X = (c_*) in the potential of the source
+ * set: 4/V sum_k k Im(L^X_i A_i), A_i = G S* exp(2 pi i k.r_i).  Each output may be NULL. */
+int tff_ewald_multipole(int dtype, const void* coords, long long natoms, const double* box_host, int kmax,
+                        double alpha, const void* charges, const void* dipoles, const void* quadrupoles,
+                        const void* c_charges, const void* c_dipoles, const void* c_quadrupoles, double* energy_out,
+                        void* potential_out, void* field_out, void* field_grad_out, void* forces_out, void* workspace,
+                        size_t workspace_bytes, tff_stream_t stream);
+
 /* ------------------------------------------------- fused energy + forces */
 #define TFF_FUSED_PAIR_OVERFLOW 1  /* cluster-pair capacity exceeded: results incomplete */
 #define TFF_FUSED_EXCL_OVERFLOW 2  /* an atom has more than 8 exclusion partners */

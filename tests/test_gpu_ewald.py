@@ -1,4 +1,4 @@
-"""Direct Ewald op (rank 0) against the float64 oracle's FFT-free sum (GPU).
+"""Direct Ewald op (point charges) against the float64 oracle's FFT-free sum (GPU).
 
 Tolerances: float64 energy 1e-11 relative, potential and force RMS relative 1e-10 (both
 from autograd of the oracle energy: dE/dq and -dE/dr); float32 1e-5 / 1e-4.
@@ -98,7 +98,7 @@ def test_ewald_rejects():
     qt = torch.zeros(3, dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError):
         torchff.ewald_long_range(ct, bt, qt, None, None, 40, 3.0)
-    with pytest.raises(NotImplementedError):
-        torchff.Ewald(3.0, 5, 1)
+    with pytest.raises(ValueError):
+        torchff.Ewald(3.0, 5, 3)
     with pytest.raises(ValueError):
         torchff.Ewald(3.0, 5, 0, use_customized_ops=False)

@@ -44,6 +44,8 @@ _SIGNATURES = {
     "tff_pme_gather_multipole": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _ll, _vp, _i, _i, _i, _i, _d, _d, _vp, _vp, _vp,
                                       _vp, _vp]),
     "tff_ewald_workspace_bytes": (_sz, [_i, _ll, _i]),
+    "tff_ewald_multipole": (_i, [_i, _vp, _ll, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp, _sz, _vp]),
     "tff_ewald": (_i, [_i, _vp, _vp, _ll, _vp, _i, _d, _vp, _vp, _vp, _vp, _sz, _vp]),
     "tff_fused_workspace_bytes": (_sz, [_i, _ll, _vp, _d, _i, _i, _i]),
     "tff_fused_energy_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i, _i, _i,

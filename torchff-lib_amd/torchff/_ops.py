@@ -505,42 +505,92 @@ class _EwaldEnergy(torch.autograd.Function):
         return -forces * s, None, pot * s, None, None
 
 
-class _EwaldAll(torch.autograd.Function):
-    """(energy, potential, field) for point charges; the field of a rank-0 evaluation is zero, as
-    the reference returns it (torchff/ewald.py:103-104); gradients flow from the energy only, as
-    in the reference's rank-0 backward (csrc/ewald/ewald.cu:450-470)."""
+def _ew_call(c, src, con, box, kmax, alpha, pot=False, field=False, fgrad=False, forces=False, energy=False):
+    """One tff_ewald_multipole evaluation: source multipoles src = (q, p, t9) (entries may be None),
+    forces contracted with con (None = the source set)."""
+    lib = L.load()
+    n = c.shape[0]
+    dt = c.dtype
+    ws = L.workspace(lib.tff_ewald_workspace_bytes(L.dtype_code(c), n, int(kmax)), c.device)
+    o_e = torch.empty((), dtype=torch.float64, device=c.device) if energy else None
+    o_pot = torch.empty(n, dtype=dt, device=c.device) if pot else None
+    o_field = torch.empty(n, 3, dtype=dt, device=c.device) if field else None
+    o_fg = torch.empty(n, 9, dtype=dt, device=c.device) if fgrad else None
+    o_f = torch.empty(n, 3, dtype=dt, device=c.device) if forces else None
+    cq, cp, ct = con if con is not None else (None, None, None)
+    rc = lib.tff_ewald_multipole(L.dtype_code(c), L.ptr(c), n, L.box_host(box), int(kmax), float(alpha),
+                                 L.ptr(src[0]), L.ptr(src[1]), L.ptr(src[2]), L.ptr(cq), L.ptr(cp), L.ptr(ct),
+                                 L.ptr(o_e), L.ptr(o_pot), L.ptr(o_field), L.ptr(o_fg), L.ptr(o_f), L.ptr(ws),
+                                 ws.numel(), L.stream_of(c))
+    L.check(rc, "ewald_long_range")
+    return o_e, o_pot, o_field, o_fg, o_f
+
+
+class _EwaldMultipole(torch.autograd.Function):
+    """Direct Ewald sum for charges, dipoles and quadrupoles (csrc/ewald/ewald.cu:9-610): returns
+    (energy, potential, field, field gradient) with the self terms; the field of a point-charge
+    call is returned as zero, as the reference's rank-0 path does (torchff/ewald.py:103-104).
+    The backward takes upstream gradients on the energy, the potential and the field: with
+    L = (q, p, theta) and M = (g_pot, -g_field, 0), d/d(q, p, theta) are the (potential, -field,
+    -field gradient/3) of L' = gE L + M, and d/dr = -(force of L in the potential of L')
+    - (force of M in the potential of L)."""
 
     @staticmethod
-    def forward(ctx, coords, box, q, kmax, alpha):
-        e, pot, forces = _ewald_call(coords, box, q, kmax, alpha)
-        ctx.save_for_backward(pot, forces)
-        ctx.mark_non_differentiable(pot)
-        field = torch.zeros_like(forces)
-        ctx.mark_non_differentiable(field)
-        return e, pot, field
+    def forward(ctx, coords, box, q, p, t9, kmax, alpha):
+        L.require_cuda(coords, box, q)
+        c, qq, pp, tt = _mp_prepare(coords, q, p, t9)
+        rank0 = pp is None and tt is None
+        e, pot, field, fg, forces = _ew_call(c, (qq, pp, tt), None, box, kmax, alpha, pot=True, field=True,
+                                             fgrad=tt is not None, forces=True, energy=True)
+        if fg is None:
+            fg = torch.zeros(c.shape[0], 9, dtype=c.dtype, device=c.device)
+        ctx.save_for_backward(c, qq, pp, tt, pot, field, fg, forces)
+        ctx.box, ctx.kmax, ctx.alpha, ctx.rank0, ctx.q_dtype = box, kmax, alpha, rank0, q.dtype
+        out_field = torch.zeros_like(field) if rank0 else field
+        if rank0:
+            ctx.mark_non_differentiable(out_field)
+        ctx.mark_non_differentiable(fg)
+        ctx.set_materialize_grads(False)
+        return e.to(c.dtype), pot, out_field, fg
 
     @staticmethod
-    def backward(ctx, ge, gp, gf):
-        pot, forces = ctx.saved_tensors
-        if ge is None:
-            return None, None, None, None, None
-        s = ge.reshape(())
-        return -forces * s, None, pot * s, None, None
+    def backward(ctx, ge, gpot, gfield, _gfg):
+        c, qq, pp, tt, pot, field, fg, forces = ctx.saved_tensors
+        has_p, has_t = pp is not None, tt is not None
+        if ctx.rank0:
+            gfield = None
+        s = (ge.reshape(()) if ge is not None else torch.zeros((), device=c.device)).to(c.dtype)
+        if gpot is None and gfield is None:
+            dcoords, dq, dp, dt9 = -forces * s, pot * s, -field * s, -fg * (s / 3.0)
+        else:
+            mq = gpot.to(c.dtype).contiguous() if gpot is not None else torch.zeros_like(qq)
+            mp_ = (-gfield.to(c.dtype)).contiguous() if gfield is not None else torch.zeros_like(c)
+            q2 = (qq * s + mq).contiguous()
+            p2 = ((pp if has_p else torch.zeros_like(c)) * s + mp_).contiguous()
+            t2 = (tt * s).contiguous() if has_t else None
+            _, pot2, field2, fg2, f1 = _ew_call(c, (q2, p2, t2), (qq, pp if has_p else torch.zeros_like(c), tt),
+                                                ctx.box, ctx.kmax, ctx.alpha, pot=True, field=True, fgrad=has_t,
+                                                forces=True)
+            _, _, _, _, f2 = _ew_call(c, (qq, pp, tt), (mq, mp_, None), ctx.box, ctx.kmax, ctx.alpha, forces=True)
+            dcoords, dq, dp = -(f1 + f2), pot2, -field2
+            dt9 = -fg2 / 3.0 if has_t else None
+        return (dcoords, None, dq.to(ctx.q_dtype), (dp if has_p else None), (dt9 if has_t else None), None, None)
 
 
-def _check_rank0(p, t, name):
-    if p is not None or t is not None:
-        raise NotImplementedError(f"{name}: only rank 0 (point charges) is provided by the gfx950 build")
+def _ewald_multipole(coords, box, q, p, t, k_max, alpha):
+    t9 = None if t is None else _as_quadrupole9(t, coords.shape[0])
+    return _EwaldMultipole.apply(coords, box, q, p, t9, int(k_max), float(alpha))
 
 
 def _ewald_long_range(coords, box, q, p, t, k_max, alpha):
-    _check_rank0(p, t, "ewald_long_range")
+    if p is not None or t is not None:
+        return _ewald_multipole(coords, box, q, p, t, k_max, alpha)[0]
     return _EwaldEnergy.apply(coords, box, q, int(k_max), float(alpha))
 
 
 def _ewald_long_range_all(coords, box, q, p, t, k_max, alpha):
-    _check_rank0(p, t, "ewald_long_range_all")
-    return _EwaldAll.apply(coords, box, q, int(k_max), float(alpha))
+    e, pot, field, _ = _ewald_multipole(coords, box, q, p, t, k_max, alpha)
+    return e, pot, field
 
 
 _auto.impl("ewald_long_range", _ewald_long_range)

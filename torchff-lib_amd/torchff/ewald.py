@@ -2,7 +2,8 @@
 
 Same functions and module as the reference (torchff/ewald.py:9-59): ``ewald_long_range``,
 ``ewald_long_range_with_fields`` and ``Ewald(alpha, kmax, rank, use_customized_ops=True,
-return_fields=True)``, backed by ``tff_ewald`` (gfx950).  Rank 0 (point charges) only; the
+return_fields=True)``, backed by ``tff_ewald`` / ``tff_ewald_multipole`` (gfx950), for point
+charges, dipoles p (N, 3) and quadrupoles t ((N, 3, 3), (N, 9) or packed (N, 6)).  The
 pure-PyTorch path (``use_customized_ops=False``) is not shipped: the float64 restatement lives in
 the oracle used by the tests.
 """
@@ -30,8 +31,8 @@ class Ewald(nn.Module):
                  return_fields: bool = True):
         super().__init__()
         require_custom(use_customized_ops, "Ewald")
-        if rank != 0:
-            raise NotImplementedError("Ewald: only rank 0 (point charges) is provided by the gfx950 build")
+        if rank not in (0, 1, 2):
+            raise ValueError(f"Ewald: rank must be 0, 1 or 2 (got {rank})")
         self.alpha = float(alpha)
         self.kmax = int(kmax)
         self.rank = rank
