@@ -178,6 +178,59 @@ int tff_ewald_multipole(int dtype, const void* coords, long long natoms, const d
                         void* potential_out, void* field_out, void* field_grad_out, void* forces_out, void* workspace,
                         size_t workspace_bytes, tff_stream_t stream);
 
+/* ------------------------------------------------------- cluster pairs */
+/* Cluster-pair neighbour list in the reference's format (torchff/nblist.py:260-344
+ * build_cluster_pairs / decode_cluster_pairs; csrc/nblist/nblist_cluster_pairs_cuda.cu:18-720) and
+ * the nonbonded terms evaluated on it (the schemas commented at
+ * csrc/nonbonded/nonbonded_interface.cpp:5-8).  Clusters are 32 consecutive slots of
+ * sorted_atom_indices; cluster_exclusions (2 x E int32, row 0 = x, row 1 = y, x <= y) lists every
+ * diagonal block and every block holding an excluded pair, with bitmask_exclusions (E x 32 uint32:
+ * word [e][l] bit k set = skip slots (32x + l, 32y + k)); interacting_clusters (M int32) and
+ * interacting_atoms (M x 32 int32, -1 padded) list, for each cluster x, the atoms of the clusters
+ * y > x outside its exclusion set that lie within the cutoff of its bounding sphere.  Decoding
+ * visits every non-excluded pair within the cutoff once.  Exclusions are a symmetric CSR over
+ * atoms (int64, at most 16 partners per atom); NULL means none.  cutoff must not exceed half the
+ * smallest box height.  Two calls: _build writes (E, M, flags) to counts_out (device, 3 int64;
+ * flags: 1 = an atom has more than 16 exclusion partners, 2 = a cluster has more than 32
+ * exclusion-partner clusters; either makes the list unusable), _emit writes the tensors with the
+ * given capacities (-1 padding past E and M) from the same workspace. */
+size_t tff_cluster_pairs_workspace_bytes(long long natoms, const double* box_host, double cutoff, double cell_size);
+int tff_cluster_pairs_build(int dtype, const void* coords, long long natoms, const double* box_host, double cutoff,
+                            double cell_size, const int64_t* excl_row_ptr, const int64_t* excl_col,
+                            long long* counts_out, void* workspace, size_t workspace_bytes, tff_stream_t stream);
+int tff_cluster_pairs_emit(int dtype, const void* coords, long long natoms, const double* box_host, double cutoff,
+                           double cell_size, const int64_t* excl_row_ptr, const int64_t* excl_col,
+                           int32_t* sorted_atom_indices, int32_t* cluster_exclusions, uint32_t* bitmask_exclusions,
+                           long long excl_capacity, int32_t* interacting_clusters, int32_t* interacting_atoms,
+                           long long row_capacity, void* workspace, size_t workspace_bytes, tff_stream_t stream);
+
+/* Workspace of _decode and _nonbonded for E exclusion blocks (columns of cluster_exclusions,
+ * padding included) and M rows. */
+size_t tff_cluster_pairs_traverse_workspace_bytes(long long natoms, long long num_exclusion_blocks,
+                                                  long long num_rows);
+/* decode_cluster_pairs: the pairs (i, j) within the cutoff (float64 minimum-image test) into
+ * pairs_out (capacity x 2 int32, -1 after the last), in entry, lane, column order; npairs_out
+ * (device, 1 int32) = the number found, which may exceed the capacity (then only capacity are
+ * written). */
+int tff_cluster_pairs_decode(int dtype, const void* coords, long long natoms, const double* box_host, double cutoff,
+                             const int32_t* sorted_atom_indices, const int32_t* cluster_exclusions,
+                             const uint32_t* bitmask_exclusions, long long num_exclusion_blocks,
+                             const int32_t* interacting_clusters, const int32_t* interacting_atoms, long long num_rows,
+                             int32_t* pairs_out, long long pair_capacity, int32_t* npairs_out, void* workspace,
+                             size_t workspace_bytes, tff_stream_t stream);
+/* compute_nonbonded_energy / forces_from_cluster_pairs: the pair term of
+ * compute_nonbonded_energy_from_atom_pairs (k q_i q_j (1/r - [1/rc]) + 4 eps s^6 (s^6 - 1), LB
+ * mixing) over the cluster pairs; outputs as tff_pair_energy (energy_out = scale * E, gradients
+ * scale * dE/dx, forces_accum += -scale * dE/dr), each may be NULL. */
+int tff_cluster_pairs_nonbonded(int dtype, const void* coords, long long natoms, const double* box_host,
+                                const void* charges, const void* sigma, const void* epsilon, double coul_constant,
+                                double cutoff, int do_shift, const int32_t* sorted_atom_indices,
+                                const int32_t* cluster_exclusions, const uint32_t* bitmask_exclusions,
+                                long long num_exclusion_blocks, const int32_t* interacting_clusters,
+                                const int32_t* interacting_atoms, long long num_rows, double scale, void* energy_out,
+                                void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon,
+                                void* forces_accum, void* workspace, size_t workspace_bytes, tff_stream_t stream);
+
 /* ------------------------------------------------- fused energy + forces */
 #define TFF_FUSED_PAIR_OVERFLOW 1  /* cluster-pair capacity exceeded: results incomplete */
 #define TFF_FUSED_EXCL_OVERFLOW 2  /* an atom has more than 8 exclusion partners */

@@ -9,10 +9,14 @@ terms and smooth PME: the same module classes, functional wrappers and
 
 from . import _ops  # noqa: F401  (registers torch.ops.torchff.*)
 from .pbc import PBC, applyPBC  # noqa: F401
-from .nblist import NeighborList, build_neighbor_list_nsquared, build_neighbor_list_cell_list  # noqa: F401
+from .nblist import (  # noqa: F401
+    NeighborList, build_neighbor_list_nsquared, build_neighbor_list_cell_list, build_cluster_pairs,
+    decode_cluster_pairs, build_neighbor_list_cluster_pairs,
+)
 from .nonbonded import (  # noqa: F401
     Nonbonded, compute_nonbonded_energy_from_atom_pairs, compute_nonbonded_forces_from_atom_pairs,
-    compute_nonbonded_ewald_energy_from_atom_pairs,
+    compute_nonbonded_ewald_energy_from_atom_pairs, compute_nonbonded_energy_from_cluster_pairs,
+    compute_nonbonded_forces_from_cluster_pairs,
 )
 from .multipoles import MultipolarInteraction  # noqa: F401
 from .vdw import Vdw, compute_lennard_jones_energy  # noqa: F401

@@ -43,6 +43,15 @@ _SIGNATURES = {
                                       _vp, _vp, _vp, _sz, _vp]),
     "tff_pme_gather_multipole": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _ll, _vp, _i, _i, _i, _i, _d, _d, _vp, _vp, _vp,
                                       _vp, _vp]),
+    "tff_cluster_pairs_workspace_bytes": (_sz, [_ll, _vp, _d, _d]),
+    "tff_cluster_pairs_build": (_i, [_i, _vp, _ll, _vp, _d, _d, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "tff_cluster_pairs_emit": (_i, [_i, _vp, _ll, _vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _ll, _vp, _sz,
+                                    _vp]),
+    "tff_cluster_pairs_traverse_workspace_bytes": (_sz, [_ll, _ll, _ll]),
+    "tff_cluster_pairs_decode": (_i, [_i, _vp, _ll, _vp, _d, _vp, _vp, _vp, _ll, _vp, _vp, _ll, _vp, _ll, _vp, _vp,
+                                      _sz, _vp]),
+    "tff_cluster_pairs_nonbonded": (_i, [_i, _vp, _ll, _vp, _vp, _vp, _vp, _d, _d, _i, _vp, _vp, _vp, _ll, _vp, _vp,
+                                         _ll, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "tff_ewald_workspace_bytes": (_sz, [_i, _ll, _i]),
     "tff_ewald_multipole": (_i, [_i, _vp, _ll, _vp, _i, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                  _vp, _sz, _vp]),

@@ -210,6 +210,208 @@ _auto.impl("compute_multipolar_energy_from_atom_pairs", _multipolar_energy)
 _auto.impl("compute_lennard_jones_energy", _lj_energy)
 
 
+# ---------------------------------------------------------------- cluster pairs
+_def.define("build_cluster_pairs(Tensor coords, Tensor box, Scalar cutoff, Tensor? exclusions, Scalar cell_size, "
+            "Scalar max_num_interacting_clusters) -> (Tensor, Tensor, Tensor, Tensor, Tensor)")
+_def.define("decode_cluster_pairs(Tensor coords, Tensor box, Tensor sorted_atom_indices, Tensor cluster_exclusions, "
+            "Tensor bitmask_exclusions, Tensor interacting_clusters, Tensor interacting_atoms, Scalar cutoff, "
+            "Scalar max_npairs, bool padding) -> (Tensor, Tensor)")
+_def.define("compute_nonbonded_energy_from_cluster_pairs(Tensor coords, Tensor box, Tensor sigma, Tensor epsilon, "
+            "Tensor charges, Scalar coul_constant, Scalar cutoff, Tensor sorted_atom_indices, "
+            "Tensor cluster_exclusions, Tensor bitmask_exclusions, Tensor interacting_clusters, "
+            "Tensor interacting_atoms, bool do_shift) -> Tensor")
+_def.define("compute_nonbonded_forces_from_cluster_pairs(Tensor coords, Tensor box, Tensor sigma, Tensor epsilon, "
+            "Tensor charges, Scalar coul_constant, Scalar cutoff, Tensor sorted_atom_indices, "
+            "Tensor cluster_exclusions, Tensor bitmask_exclusions, Tensor interacting_clusters, "
+            "Tensor interacting_atoms, Tensor(a!) forces) -> ()")
+
+
+def _exclusions_csr(exclusions, n, device):
+    """Symmetric int64 CSR of a (2, E) (or (E, 2)) COO exclusion list, as NeighborList builds it."""
+    if exclusions is None or exclusions.numel() == 0:
+        return None, None
+    e = exclusions.detach().to(torch.int64)
+    if e.ndim != 2 or 2 not in e.shape:
+        raise ValueError(f"build_cluster_pairs: exclusions must be (2, E) or (E, 2), got {tuple(e.shape)}")
+    pairs = e.t() if e.shape[0] == 2 else e
+    pairs = pairs[(pairs[:, 0] >= 0) & (pairs[:, 1] >= 0)]
+    allp = torch.cat([pairs, pairs.flip(1)], dim=0)
+    keys = torch.unique(allp[:, 0] * n + allp[:, 1])
+    rows, col = keys // n, keys % n
+    counts = torch.zeros(n, dtype=torch.long, device=device)
+    counts.scatter_add_(0, rows, torch.ones_like(rows))
+    row_ptr = torch.zeros(n + 1, dtype=torch.long, device=device)
+    row_ptr[1:] = torch.cumsum(counts, dim=0)
+    return row_ptr.contiguous(), col.contiguous()
+
+
+def _build_cluster_pairs(coords, box, cutoff, exclusions, cell_size, max_num_interacting_clusters):
+    """build_cluster_pairs (torchff/nblist.py:260-286): the five tensors of the cluster-pair list.
+    With max_num_interacting_clusters <= 0 the rows are sized exactly; otherwise that many rows
+    are returned (-1 padded) and a larger need raises, as the capacity overflow of the pair lists."""
+    L.require_cuda(coords, box, exclusions)
+    lib = L.load()
+    c = coords.detach().contiguous()
+    n = c.shape[0]
+    bh = L.box_host(box)
+    rp, col = _exclusions_csr(exclusions, n, c.device)
+    nbytes = lib.tff_cluster_pairs_workspace_bytes(n, bh, float(cutoff), float(cell_size))
+    if nbytes == 0:
+        L.check(lib.tff_cluster_pairs_build(L.dtype_code(c), L.ptr(c), n, bh, float(cutoff), float(cell_size), None,
+                                            None, None, None, 0, L.stream_of(c)), "build_cluster_pairs")
+    ws = L.workspace(nbytes, c.device)
+    counts = torch.zeros(3, dtype=torch.int64, device=c.device)
+    rc = lib.tff_cluster_pairs_build(L.dtype_code(c), L.ptr(c), n, bh, float(cutoff), float(cell_size), L.ptr(rp),
+                                     L.ptr(col), L.ptr(counts), L.ptr(ws), ws.numel(), L.stream_of(c))
+    L.check(rc, "build_cluster_pairs")
+    E, M, flags = (int(v) for v in counts.cpu())
+    if flags & 1:
+        raise RuntimeError("build_cluster_pairs: an atom has more than 16 exclusion partners")
+    if flags & 2:
+        raise RuntimeError("build_cluster_pairs: a cluster has more than 32 exclusion-partner clusters")
+    cap = int(max_num_interacting_clusters)
+    if cap > 0 and M > cap:
+        raise RuntimeError(f"build_cluster_pairs: {M} interacting rows exceed max_num_interacting_clusters={cap}")
+    rows = cap if cap > 0 else M
+    dev = c.device
+    sorted_idx = torch.empty(n, dtype=torch.int32, device=dev)
+    coo = torch.empty((2, E), dtype=torch.int32, device=dev)
+    mask = torch.empty((E, 32), dtype=torch.uint32, device=dev)
+    ic = torch.empty(rows, dtype=torch.int32, device=dev)
+    ia = torch.empty((rows, 32), dtype=torch.int32, device=dev)
+    rc = lib.tff_cluster_pairs_emit(L.dtype_code(c), L.ptr(c), n, bh, float(cutoff), float(cell_size), L.ptr(rp),
+                                    L.ptr(col), L.ptr(sorted_idx), L.ptr(coo), L.ptr(mask), E, L.ptr(ic), L.ptr(ia),
+                                    rows, L.ptr(ws), ws.numel(), L.stream_of(c))
+    L.check(rc, "build_cluster_pairs")
+    return sorted_idx, coo, mask, ic, ia
+
+
+def _cluster_entries(sorted_atom_indices, cluster_exclusions, bitmask_exclusions, interacting_clusters,
+                     interacting_atoms):
+    si = sorted_atom_indices.detach().to(torch.int32).contiguous()
+    coo = cluster_exclusions.detach().to(torch.int32).contiguous()
+    if coo.ndim != 2 or coo.shape[0] != 2:
+        raise ValueError("cluster_exclusions must be (2, E)")
+    E = coo.shape[1]
+    mask = bitmask_exclusions.detach().contiguous()
+    if mask.dtype not in (torch.uint32, torch.int32):
+        raise ValueError("bitmask_exclusions must be uint32 (or int32 of the same bits)")
+    if mask.shape[0] < E or (E > 0 and mask.shape[1] != 32):
+        raise ValueError("bitmask_exclusions must be (E, 32)")
+    ic = interacting_clusters.detach().to(torch.int32).contiguous()
+    ia = interacting_atoms.detach().to(torch.int32).contiguous()
+    M = ic.shape[0]
+    if ia.shape[0] < M or (M > 0 and ia.shape[1] != 32):
+        raise ValueError("interacting_atoms must be (M, 32)")
+    return si, coo, E, mask, ic, ia, M
+
+
+def _decode_cluster_pairs(coords, box, sorted_atom_indices, cluster_exclusions, bitmask_exclusions,
+                          interacting_clusters, interacting_atoms, cutoff, max_npairs, padding):
+    """decode_cluster_pairs (torchff/nblist.py:289-316): (pairs (P, 2) int32, npairs (1,) int32)."""
+    L.require_cuda(coords, box, sorted_atom_indices, cluster_exclusions, bitmask_exclusions, interacting_clusters,
+                   interacting_atoms)
+    lib = L.load()
+    c = coords.detach().contiguous()
+    n = c.shape[0]
+    bh = L.box_host(box)
+    si, coo, E, mask, ic, ia, M = _cluster_entries(sorted_atom_indices, cluster_exclusions, bitmask_exclusions,
+                                                   interacting_clusters, interacting_atoms)
+    ws = L.workspace(lib.tff_cluster_pairs_traverse_workspace_bytes(n, E, M), c.device)
+    npairs = torch.zeros(1, dtype=torch.int32, device=c.device)
+
+    def run(pairs, cap):
+        rc = lib.tff_cluster_pairs_decode(L.dtype_code(c), L.ptr(c), n, bh, float(cutoff), L.ptr(si), L.ptr(coo),
+                                          L.ptr(mask), E, L.ptr(ic), L.ptr(ia), M, L.ptr(pairs), cap, L.ptr(npairs),
+                                          L.ptr(ws), ws.numel(), L.stream_of(c))
+        L.check(rc, "decode_cluster_pairs")
+
+    cap = int(max_npairs)
+    if cap < 0:
+        run(None, 0)  # count only: size the list exactly
+        cap = int(npairs.item())
+    pairs = torch.empty((cap, 2), dtype=torch.int32, device=c.device)
+    run(pairs, cap)
+    if padding:
+        return pairs, npairs
+    found = int(npairs.item())
+    if found > cap:
+        raise RuntimeError(f"Too many neighbor pairs found. Maximum is {cap} but found {found}")
+    return pairs[:found], npairs
+
+
+def _cluster_nonbonded_call(coords, box, sigma, epsilon, charges, coul_constant, cutoff, entries, do_shift,
+                            want=(), forces=None):
+    L.require_cuda(coords, box, sigma, epsilon, charges, forces)
+    lib = L.load()
+    dt = coords.dtype
+    c = coords.detach().contiguous()
+    n = c.shape[0]
+    si, coo, E, mask, ic, ia, M = entries
+    q, sg, ep = (t.detach().to(dt).contiguous() for t in (charges, sigma, epsilon))
+    energy = torch.empty((1,), dtype=dt, device=c.device)
+    grads = {name: torch.empty_like(ref) for name, ref in (("coords", c), ("charges", q), ("sigma", sg),
+                                                           ("epsilon", ep)) if name in want}
+    ws = L.workspace(lib.tff_cluster_pairs_traverse_workspace_bytes(n, E, M), c.device)
+    rc = lib.tff_cluster_pairs_nonbonded(L.dtype_code(c), L.ptr(c), n, L.box_host(box), L.ptr(q), L.ptr(sg),
+                                         L.ptr(ep), float(coul_constant), float(cutoff), int(bool(do_shift)),
+                                         L.ptr(si), L.ptr(coo), L.ptr(mask), E, L.ptr(ic), L.ptr(ia), M, 1.0,
+                                         L.ptr(energy), L.ptr(grads.get("coords")), L.ptr(grads.get("charges")),
+                                         L.ptr(grads.get("sigma")), L.ptr(grads.get("epsilon")), L.ptr(forces),
+                                         L.ptr(ws), ws.numel(), L.stream_of(c))
+    L.check(rc, "nonbonded from cluster pairs")
+    return energy, grads
+
+
+class _ClusterNonbonded(torch.autograd.Function):
+    """Energy on the cluster pairs; gradients formed in the forward and scaled in the backward,
+    like the atom-pair op (csrc/nonbonded/nonbonded_atom_pairs_cuda.cu:143-260)."""
+
+    @staticmethod
+    def forward(ctx, coords, box, sigma, epsilon, charges, coul, cutoff, do_shift, si, coo, mask, ic, ia):
+        want = [k for k, t in (("coords", coords), ("charges", charges), ("sigma", sigma), ("epsilon", epsilon))
+                if t.requires_grad]
+        entries = _cluster_entries(si, coo, mask, ic, ia)
+        e, g = _cluster_nonbonded_call(coords, box, sigma, epsilon, charges, coul, cutoff, entries, do_shift,
+                                       want=want)
+        ctx.names = list(g.keys())
+        ctx.save_for_backward(*[g[k] for k in ctx.names])
+        return e
+
+    @staticmethod
+    def backward(ctx, ge):
+        saved = dict(zip(ctx.names, ctx.saved_tensors))
+        s = ge.reshape(())
+
+        def gr(name):
+            return saved[name] * s if name in saved else None
+
+        return (gr("coords"), None, gr("sigma"), gr("epsilon"), gr("charges")) + (None,) * 8
+
+
+def _cluster_energy(coords, box, sigma, epsilon, charges, coul_constant, cutoff, sorted_atom_indices,
+                    cluster_exclusions, bitmask_exclusions, interacting_clusters, interacting_atoms, do_shift):
+    return _ClusterNonbonded.apply(coords, box, sigma, epsilon, charges, float(coul_constant), float(cutoff),
+                                   bool(do_shift), sorted_atom_indices, cluster_exclusions, bitmask_exclusions,
+                                   interacting_clusters, interacting_atoms)
+
+
+def _cluster_forces(coords, box, sigma, epsilon, charges, coul_constant, cutoff, sorted_atom_indices,
+                    cluster_exclusions, bitmask_exclusions, interacting_clusters, interacting_atoms, forces):
+    if forces.dtype != coords.dtype or not forces.is_contiguous() or forces.shape != coords.shape:
+        raise RuntimeError("forces must be a contiguous tensor shaped and typed like coords")
+    entries = _cluster_entries(sorted_atom_indices, cluster_exclusions, bitmask_exclusions, interacting_clusters,
+                               interacting_atoms)
+    _cluster_nonbonded_call(coords, box, sigma, epsilon, charges, coul_constant, cutoff, entries, True,
+                            forces=forces)
+
+
+_cuda.impl("build_cluster_pairs", _build_cluster_pairs)
+_cuda.impl("decode_cluster_pairs", _decode_cluster_pairs)
+_auto.impl("compute_nonbonded_energy_from_cluster_pairs", _cluster_energy)
+_cuda.impl("compute_nonbonded_forces_from_cluster_pairs", _cluster_forces)
+
+
 # ---------------------------------------------------------------- PME
 _def.define("pme_long_range(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar K, Scalar alpha, "
             "Tensor xmoduli, Tensor ymoduli, Tensor zmoduli, int order=6) -> Tensor")

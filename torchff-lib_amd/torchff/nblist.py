@@ -106,3 +106,30 @@ def _functional(algo, coords, box, cutoff, max_npairs, padding, excl_row_ptr, ex
             raise RuntimeError(f"Too many neighbor pairs found. Maximum is {pairs.shape[0]} but found {found}")
         pairs = pairs[:found]
     return pairs, npairs
+
+
+def build_cluster_pairs(coords: torch.Tensor, box: torch.Tensor, cutoff: float,
+                        exclusions: Optional[torch.Tensor] = None, cell_size: float = 0.4,
+                        max_num_interacting_clusters: int = -1):
+    """Cluster-pair list (torchff/nblist.py:260-286): (sorted_atom_indices, cluster_exclusions,
+    bitmask_exclusions, interacting_clusters, interacting_atoms); exclusions as a (2, E) COO list."""
+    return torch.ops.torchff.build_cluster_pairs(coords, box, cutoff, exclusions, cell_size,
+                                                 max_num_interacting_clusters)
+
+
+def decode_cluster_pairs(coords: torch.Tensor, box: torch.Tensor, sorted_atom_indices, cluster_exclusions,
+                         bitmask_exclusions, interacting_clusters, interacting_atoms, cutoff: float,
+                         max_npairs: int = -1, padding: bool = False):
+    """Atom pairs of a cluster-pair list within the cutoff (torchff/nblist.py:289-316): (pairs, npairs)."""
+    return torch.ops.torchff.decode_cluster_pairs(coords, box, sorted_atom_indices, cluster_exclusions,
+                                                  bitmask_exclusions, interacting_clusters, interacting_atoms,
+                                                  cutoff, max_npairs, padding)
+
+
+def build_neighbor_list_cluster_pairs(coords: torch.Tensor, box: torch.Tensor, cutoff: float,
+                                      exclusions: Optional[torch.Tensor], cell_size: float = 0.45,
+                                      max_num_interacting_clusters: int = -1, max_npairs: int = -1,
+                                      padding: bool = False):
+    """Pair list through the cluster pairs (torchff/nblist.py:319-344, without its debug prints)."""
+    nbl = build_cluster_pairs(coords, box, cutoff, exclusions, cell_size, max_num_interacting_clusters)
+    return decode_cluster_pairs(coords, box, *nbl, cutoff, max_npairs, padding)

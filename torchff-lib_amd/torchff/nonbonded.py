@@ -43,6 +43,25 @@ def compute_nonbonded_ewald_energy_from_atom_pairs(coords, pairs, box, sigma, ep
         coords, pairs, box, sigma, epsilon, charges, coul_constant, cutoff, ewald_alpha, pairs_excl)
 
 
+def compute_nonbonded_energy_from_cluster_pairs(coords, box, sigma, epsilon, charges, coul_constant, cutoff,
+                                                sorted_atom_indices, cluster_exclusions, bitmask_exclusions,
+                                                interacting_clusters, interacting_atoms, do_shift=True):
+    """The atom-pair nonbonded energy evaluated on a cluster-pair list (the schema commented at
+    csrc/nonbonded/nonbonded_interface.cpp:7), differentiable in coords, sigma, epsilon, charges."""
+    return torch.ops.torchff.compute_nonbonded_energy_from_cluster_pairs(
+        coords, box, sigma, epsilon, charges, coul_constant, cutoff, sorted_atom_indices, cluster_exclusions,
+        bitmask_exclusions, interacting_clusters, interacting_atoms, do_shift)
+
+
+def compute_nonbonded_forces_from_cluster_pairs(coords, box, sigma, epsilon, charges, coul_constant, cutoff,
+                                                sorted_atom_indices, cluster_exclusions, bitmask_exclusions,
+                                                interacting_clusters, interacting_atoms, forces):
+    """Adds the nonbonded forces on a cluster-pair list into ``forces`` (nonbonded_interface.cpp:8)."""
+    torch.ops.torchff.compute_nonbonded_forces_from_cluster_pairs(
+        coords, box, sigma, epsilon, charges, coul_constant, cutoff, sorted_atom_indices, cluster_exclusions,
+        bitmask_exclusions, interacting_clusters, interacting_atoms, forces)
+
+
 class Nonbonded(nn.Module):
     """Fused fixed-charge nonbonded interaction (torchff/nonbonded.py:99-141)."""
 
