@@ -327,6 +327,51 @@ def pairs_within(w, dev, r):
     return int(n.item())
 
 
+def cluster_pair_results(w, dev, reps=20):
+    """The nonbonded forces op (shifted Coulomb + LJ, float32 as the reference's MD loop uses it) on a
+    cluster-pair list against the same op on a (P, 2) int64 pair list from the cell list, both lists
+    at the cutoff with the intramolecular exclusions: list build and force times, list bytes."""
+    import torchff
+    dt = torch.float32
+    rc = w["cutoff"]
+    x = torch.tensor(w["coords"], dtype=dt, device=dev)
+    box = torch.tensor(w["box"], dtype=dt, device=dev)
+    q, sig, eps = (torch.tensor(w[k], dtype=dt, device=dev) for k in ("charges", "sigma", "epsilon"))
+    excl = torch.tensor(w["exclusions"], device=dev)
+    k = 138.935456
+    f = torch.zeros_like(x)
+    nbl = None
+
+    def build_cp():
+        nonlocal nbl
+        nbl = torchff.build_cluster_pairs(x, box, rc, excl, 0.6)
+        return nbl
+
+    t_cp_build, _ = time_steps(build_cp, 3, 1)
+    t_cp_forces, _ = time_steps(
+        lambda: torchff.compute_nonbonded_forces_from_cluster_pairs(x, box, sig, eps, q, k, rc, *nbl, f), reps, 2)
+    nl = torchff.NeighborList(w["natoms"], exclusions=excl, algorithm="cell_list")
+    cap = int(w["natoms"] * 4.0 / 3.0 * math.pi * rc ** 3 * 100.0 / 2.0 * 1.3) + 1024
+    pairs = None
+
+    def build_ap():
+        nonlocal pairs
+        pairs, npairs = nl(x, box, rc, cap, padding=True)
+        return pairs
+
+    t_ap_build, _ = time_steps(build_ap, 3, 1)
+    t_ap_forces, _ = time_steps(
+        lambda: torchff.compute_nonbonded_forces_from_atom_pairs(x, pairs, box, sig, eps, q, k, rc, f), reps, 2)
+    E, M = nbl[1].shape[1], nbl[3].shape[0]
+    cp_bytes = 4 * w["natoms"] + 8 * E + 128 * E + 4 * M + 128 * M
+    npairs = pairs_within(w, dev, rc)
+    return {"note": "float32, cutoff list without skin; build times include the host read-back of the sizes",
+            "cluster_pairs": {"build_ms": t_cp_build * 1e3, "forces_ms": t_cp_forces * 1e3, "list_bytes": cp_bytes,
+                              "exclusion_blocks": E, "rows": M},
+            "atom_pairs": {"build_ms": t_ap_build * 1e3, "forces_ms": t_ap_forces * 1e3,
+                           "list_bytes": 16 * npairs, "pairs": npairs}}
+
+
 def rms_rel(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
 
@@ -441,6 +486,8 @@ def main():
                       for st, b in sb.items()}}
     if head == "mixed" and not args.no_serving:
         out["serving"] = serving_results(args, dev)
+    if head == "mixed":
+        out["nonbonded_list_formats"] = cluster_pair_results(w, dev)
     if not args.no_cpu_baseline:
         from oracle import composite
         threads = os.cpu_count() or 1
