@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--only", default="", help="comma list of paths to run (mixed,fp64,dropin)")
     ap.add_argument("--differentiation", action="store_true",
                     help="time force-matching steps (E, F, loss, gradients wrt coords, q, sigma, eps) instead")
+    ap.add_argument("--widened", action="store_true",
+                    help="multipolar PME (ranks 0-2) and the direct Ewald op, energy + gradients per call")
     ap.add_argument("--serving", action="store_true",
                     help="serve 4096 Poisson requests (20-1000 waters) batched and one system per launch instead")
     ap.add_argument("--requests", type=int, default=4096)
@@ -230,6 +232,64 @@ def differentiation_bench(args, w, dev):
     }))
 
 
+def widened_bench(args, w, dev):
+    """Reciprocal space for point dipoles and quadrupoles (PME ranks 1 and 2 beside rank 0) on the
+    workload's box and grid, and the direct Ewald op on the 900-atom box: one call = energy and the
+    gradients of every input (coords, q, p, theta).  Dipoles N(0, 0.02) e nm, quadrupoles
+    symmetric traceless N(0, 0.01) e nm^2."""
+    import torchff
+    from torchff.workloads import make_workload
+
+    def inputs(wk, dt, rank, seed):
+        rng = np.random.default_rng(seed)
+        n = wk["natoms"]
+        t = lambda a: torch.tensor(a, dtype=dt, device=dev)
+        x, box, q = t(wk["coords"]), t(wk["box"]), t(wk["charges"])
+        p = t(rng.normal(0.0, 0.02, (n, 3))) if rank >= 1 else None
+        th = None
+        if rank >= 2:
+            a = rng.normal(0.0, 0.01, (n, 3, 3))
+            a = 0.5 * (a + a.transpose(0, 2, 1))
+            a -= np.trace(a, axis1=1, axis2=2)[:, None, None] * np.eye(3)[None] / 3.0
+            th = t(a)
+        return x, box, q, p, th
+
+    def run(mod, x, box, q, p, th, steps, names):
+        leaves = [v.requires_grad_(True) for v in (x, q, p, th) if v is not None]
+
+        def step():
+            E = mod(x, box, q, p, th)
+            return torch.autograd.grad(E, leaves)
+
+        sec, _ = time_steps(step, steps, 2)
+        kt = kernel_times(step, steps, names) if names else {}
+        return sec, {k: v["avg_ms"] for k, v in kt.items()}
+
+    res = {}
+    for dt, tag in ((torch.float32, "f32"), (torch.float64, "f64")):
+        for rank in (0, 1, 2):
+            m = torchff.PME(w["alpha"], w["K"], rank, order=w["order"]).to(device=dev, dtype=dt)
+            sec, kt = run(m, *inputs(w, dt, rank, rank), max(5, args.steps // 5),
+                          ["pme_spread", "pme_fft", "pme_fft_conv", "pme_gather"])
+            res[f"pme_rank{rank}_{tag}"] = {"ms": sec * 1e3, "evals_per_s": 1.0 / sec, "kernels_ms": kt}
+    w9 = make_workload("water900")
+    for rank in (0, 1, 2):
+        m = torchff.Ewald(w9["alpha"], 10, rank, return_fields=False)
+        sec, _ = run(m, *inputs(w9, torch.float64, rank, 10 + rank), 5, None)
+        res[f"ewald_rank{rank}_f64_water900_kmax10"] = {"ms": sec * 1e3, "evals_per_s": 1.0 / sec}
+    head = res["pme_rank2_f32"]
+    print(json.dumps({
+        "metric": f"PME energy + gradient evals/s, point charges + dipoles + quadrupoles, {w['natoms'] // 1000}k "
+                  "atoms",
+        "value": head["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": max(5, args.steps // 5),
+        "warmup": 2, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload} coordinates and charges, random dipoles and traceless quadrupoles, "
+                               f"PME {w['K']}^3 order {w['order']}; direct Ewald on water900 with kmax 10",
+                   "path": "torchff.PME(rank) / torchff.Ewald(rank) drop-in ops, autograd backward"},
+        "results": res}))
+
+
 def serving_results(args, dev):
     """Request serving: args.requests water boxes of 20-1000 waters (log-uniform, 1 g/cm^3, rc 0.9 nm,
     PME spacing <= 0.1 nm rounded to 2^a 3^b 5^c, order 5, mixed precision) arriving as a Poisson
@@ -396,6 +456,9 @@ def main():
     w = make_workload(args.workload)
     if args.differentiation:
         differentiation_bench(args, w, dev)
+        return
+    if args.widened:
+        widened_bench(args, w, dev)
         return
     res = {}
     outs = {}
