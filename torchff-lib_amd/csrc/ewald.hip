@@ -100,24 +100,25 @@ __global__ __launch_bounds__(EW_WAVES * TFF_WAVE) void k_ewald_structure(const T
   const int mz = lane - kmax;
   // half space: m_x = m_y = 0 keeps m_z > 0 only
   const bool valid = lane_ok && !(mx == 0 && my == 0 && mz <= 0);
-  double e = 0;
+  double e = 0, g = 0;
   if (valid) {
     const double kx = box.inv[0] * mx + box.inv[1] * my + box.inv[2] * mz;
     const double ky = box.inv[3] * mx + box.inv[4] * my + box.inv[5] * mz;
     const double kz = box.inv[6] * mx + box.inv[7] * my + box.inv[8] * mz;
     const double k2 = kx * kx + ky * ky + kz * kz;
-    const double g = exp(-M_PI * M_PI * k2 / (alpha * alpha)) / k2;
+    g = exp(-M_PI * M_PI * k2 / (alpha * alpha)) / k2;
     e = g * (sr * sr + si * si);
   }
+  // G S: the per-atom pass then needs no transcendental per k
   if (lane_ok) {
-    S[2 * ((size_t)line * W + lane)] = valid ? sr : 0.0;
-    S[2 * ((size_t)line * W + lane) + 1] = valid ? si : 0.0;
+    S[2 * ((size_t)line * W + lane)] = g * sr;
+    S[2 * ((size_t)line * W + lane) + 1] = g * si;
   }
   e = tff_wave_sum(e);
   if (lane == 0) eline[line] = e;
 }
 
-// potential (incl. the self term) and force of one atom per thread; energy by one thread
+// potential (incl. the self term) and force of one atom per thread from G S; energy by one thread
 template <typename T>
 __global__ __launch_bounds__(256) void k_ewald_atoms(const T* __restrict__ q, long long n, int kmax, TffBox box,
                                                      double alpha, int nlines, const CX<T>* __restrict__ tab,
@@ -152,11 +153,9 @@ __global__ __launch_bounds__(256) void k_ewald_atoms(const T* __restrict__ q, lo
       const double kx = box.inv[0] * mx + box.inv[1] * my + box.inv[2] * mz;
       const double ky = box.inv[3] * mx + box.inv[4] * my + box.inv[5] * mz;
       const double kz = box.inv[6] * mx + box.inv[7] * my + box.inv[8] * mz;
-      const double k2 = kx * kx + ky * ky + kz * kz;
-      const double g = exp(-M_PI * M_PI * k2 / (alpha * alpha)) / k2;
-      const double Sr = S[2 * ((size_t)line * W + h)], Si = S[2 * ((size_t)line * W + h) + 1];
-      ph += g * (Sr * c + Si * s);
-      const double fi = g * (Sr * s - Si * c);
+      const double Sr = S[2 * ((size_t)line * W + h)], Si = S[2 * ((size_t)line * W + h) + 1];  // G S
+      ph += Sr * c + Si * s;
+      const double fi = Sr * s - Si * c;
       fx += fi * kx;
       fy += fi * ky;
       fz += fi * kz;
@@ -257,15 +256,15 @@ __global__ __launch_bounds__(EW_WAVES * TFF_WAVE) void k_ewald_structure_mp(EwMp
     si += red[w][1][lane];
   }
   const bool valid = lane_ok && !(mx == 0 && my == 0 && mz <= 0);
-  double e = 0;
+  double e = 0, g = 0;
   if (valid) {
     const double k2 = kx * kx + ky * ky + kz * kz;
-    const double g = exp(-M_PI * M_PI * k2 / (alpha * alpha)) / k2;
+    g = exp(-M_PI * M_PI * k2 / (alpha * alpha)) / k2;
     e = g * (sr * sr + si * si);
   }
   if (lane_ok) {
-    S[2 * ((size_t)line * W + lane)] = valid ? sr : 0.0;
-    S[2 * ((size_t)line * W + lane) + 1] = valid ? si : 0.0;
+    S[2 * ((size_t)line * W + lane)] = g * sr;
+    S[2 * ((size_t)line * W + lane) + 1] = g * si;
   }
   e = tff_wave_sum(e);
   if (lane == 0) eline[line] = e;
@@ -320,10 +319,8 @@ __global__ __launch_bounds__(256) void k_ewald_atoms_mp(EwMp src, EwMp con, long
       const double c = ax * tz[h].x - ay * tz[h].y, s = ax * tz[h].y + ay * tz[h].x;
       double kx, ky, kz;
       ew_k(box, mx, my, mz, kx, ky, kz);
-      const double k2 = kx * kx + ky * ky + kz * kz;
-      const double g = exp(-M_PI * M_PI * k2 / (alpha * alpha)) / k2;
-      const double Sr = S[2 * ((size_t)line * W + h)], Si = S[2 * ((size_t)line * W + h) + 1];
-      const double Ar = g * (Sr * c + Si * s), Ai = g * (Sr * s - Si * c);
+      const double Sr = S[2 * ((size_t)line * W + h)], Si = S[2 * ((size_t)line * W + h) + 1];  // G S
+      const double Ar = Sr * c + Si * s, Ai = Sr * s - Si * c;
       sar += Ar;
       fa[0] += kx * Ai;
       fa[1] += ky * Ai;

@@ -8,9 +8,11 @@
 // the hard-wired spline order 6 to orders 3..8.
 //
 // Pipeline (one K0 x K1 x K2 grid, layout (x, y, z) with z fastest):
-//   1. spread      one thread per atom: s = K (r . B^-1), m0 = ceil(s - (p mod 2)/2),
-//                  weights M_p(t + k), t = m0 - s + p/2 - floor(p/2) in [0, 1), onto
-//                  grid points m0 - floor(p/2) + k (k = 0..p-1), float64 atomics.
+//   1. spread      s = K (r . B^-1), m0 = ceil(s - (p mod 2)/2), weights M_p(t + k),
+//                  t = m0 - s + p/2 - floor(p/2) in [0, 1), onto grid points m0 - floor(p/2) + k
+//                  (k = 0..p-1): atoms binned by the 8^3 brick of their stencil base, one
+//                  workgroup per brick summing the reaching atoms in LDS (float64) and writing
+//                  its brick.
 //   2. FFT z       (forward; reads the float64 charge grid, writes complex T)
 //   3. FFT y       (forward)
 //   4. FFT x  +  influence function G(k) = 4 pi exp(-k^2/4a^2) / (V k^2 |theta|^2)
@@ -96,80 +98,199 @@ __device__ __forceinline__ void lattice_multipoles(const Multipoles& mp, long lo
   }
 }
 
-template <typename T, int P, int LVL>
-__global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Multipoles mp, long long n, PmeGeom g,
-                                                double* __restrict__ grid) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+// Owner-brick spread.  The grid is cut into bricks of SB^3 points; the atoms are binned by the
+// brick of their stencil base (a counting sort: k_brick_bin, k_brick_scan, k_brick_scatter); one
+// workgroup per brick then takes the atoms of every brick whose stencils can reach it (per axis the
+// bricks holding bases in [start - P + 1, end], at most 4), adds their in-brick contributions in LDS
+// (float64) and writes its brick once.  No global atomics, and no clearing of the grid: every
+// point is written by its owner.
+constexpr int SB = 8;
+constexpr int SB_SRC = 4;  // source bricks per axis
+
+__device__ __forceinline__ void atom_base(const double* r, const PmeGeom& g, int a, int P, int& base, double& t) {
+  const double s = (r[0] * g.inv[a] + r[1] * g.inv[3 + a] + r[2] * g.inv[6 + a]) * (double)g.K[a];
+  anchor_axis(s, g.K[a], P, base, t);
+}
+
+template <typename T>
+__global__ void k_brick_bin(const T* __restrict__ coords, long long n, PmeGeom g, int* __restrict__ brick_of,
+                            int* __restrict__ count) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double r0 = (double)coords[3 * i], r1 = (double)coords[3 * i + 1], r2 = (double)coords[3 * i + 2];
-  constexpr int PD = LVL >= 2 ? P : 1;
-  T Mv[3][P], dM[3][P], d2M[3][PD];
-  int base[3];
-#pragma unroll
+  const double r[3] = {(double)coords[3 * i], (double)coords[3 * i + 1], (double)coords[3 * i + 2]};
+  int bb[3];
   for (int a = 0; a < 3; ++a) {
-    double s = (r0 * g.inv[a] + r1 * g.inv[3 + a] + r2 * g.inv[6 + a]) * (double)g.K[a];
+    int base;
     double t;
-    anchor_axis(s, g.K[a], P, base[a], t);
-    if constexpr (LVL >= 2)
-      spline_weights2<T, P>((T)t, Mv[a], dM[a], d2M[a]);
-    else
-      spline_weights<T, P>((T)t, Mv[a], dM[a]);
+    atom_base(r, g, a, g.order, base, t);
+    bb[a] = base / SB;
   }
-  const int K1 = g.K[1], K2 = g.K[2];
-  if constexpr (LVL == 0) {
-    const T qi = ((const T*)mp.q)[i];
-    int gx = base[0];
-    for (int a = 0; a < P; ++a) {
-      int gy = base[1];
-      const T wa = qi * Mv[0][a];
-      for (int b = 0; b < P; ++b) {
-        const T wab = wa * Mv[1][b];
-        double* row = grid + ((long long)gx * K1 + gy) * K2;
-        int gz = base[2];
-#pragma unroll
-        for (int c = 0; c < P; ++c) {
-          atomicAdd(&row[gz], (double)(wab * Mv[2][c]));
-          gz = gz + 1 == K2 ? 0 : gz + 1;
-        }
-        gy = gy + 1 == K1 ? 0 : gy + 1;
+  const int nb1 = (g.K[1] + SB - 1) / SB, nb2 = (g.K[2] + SB - 1) / SB;
+  const int id = (bb[0] * nb1 + bb[1]) * nb2 + bb[2];
+  brick_of[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+// exclusive scan of count[0..m) into start[0..m], one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_brick_scan(const int* __restrict__ count, int m, int* __restrict__ start) {
+  __shared__ int s_tot[1024];
+  const int t = threadIdx.x;
+  const int per = (m + 1023) / 1024;
+  const int b = t * per, e = b + per < m ? b + per : m;
+  int sum = 0;
+  for (int k = b; k < e; ++k) sum += count[k];
+  s_tot[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? s_tot[t - o] : 0;
+    __syncthreads();
+    s_tot[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? s_tot[t - 1] : 0;
+  for (int k = b; k < e; ++k) {
+    start[k] = run;
+    run += count[k];
+  }
+  if (t == 1023) start[m] = s_tot[1023];
+}
+
+__global__ void k_brick_scatter(long long n, const int* __restrict__ brick_of, const int* __restrict__ start,
+                                int* __restrict__ cursor, int* __restrict__ list) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = brick_of[i];
+  list[start[id] + atomicAdd(&cursor[id], 1)] = (int)i;
+}
+
+__global__ void k_zero_ints(int* __restrict__ p, int m) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) p[k] = 0;
+}
+
+template <typename T, int P, int LVL>
+__global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Multipoles mp, PmeGeom g,
+                                                const int* __restrict__ start, const int* __restrict__ list,
+                                                double* __restrict__ grid) {
+  __shared__ double s_b[SB * SB * SB];
+  const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
+  const int nb[3] = {(K0 + SB - 1) / SB, (K1 + SB - 1) / SB, (K2 + SB - 1) / SB};
+  const int bid = blockIdx.x;
+  const int B[3] = {bid / (nb[1] * nb[2]), (bid / nb[2]) % nb[1], bid % nb[2]};
+  int B0[3], Bn[3];
+  for (int a = 0; a < 3; ++a) {
+    B0[a] = B[a] * SB;
+    Bn[a] = g.K[a] - B0[a] < SB ? g.K[a] - B0[a] : SB;
+  }
+  for (int k = threadIdx.x; k < SB * SB * SB; k += blockDim.x) s_b[k] = 0.0;
+  // source bricks: per axis those holding a stencil base in [B0 - P + 1, B0 + Bn - 1] (periodic);
+  // their atom ranges are concatenated so that every thread takes atoms of the whole set
+  __shared__ int s_a0[SB_SRC * SB_SRC * SB_SRC], s_pref[SB_SRC * SB_SRC * SB_SRC + 1], s_nc;
+  if (threadIdx.x == 0) {
+    int src[3][SB_SRC], ns[3];
+    for (int a = 0; a < 3; ++a) {
+      ns[a] = 0;
+      for (int v = B0[a] - (P - 1); v <= B0[a] + Bn[a] - 1; ++v) {
+        int w = v % g.K[a];
+        w = w < 0 ? w + g.K[a] : w;
+        const int sb = w / SB;
+        bool seen = false;
+        for (int k = 0; k < ns[a]; ++k) seen |= src[a][k] == sb;
+        if (!seen && ns[a] < SB_SRC) src[a][ns[a]++] = sb;
       }
-      gx = gx + 1 == g.K[0] ? 0 : gx + 1;
     }
-  } else {
-    double pud[3], tcd[6];
-    lattice_multipoles<T>(mp, i, g, pud, tcd);
-    const T qi = mp.q ? ((const T*)mp.q)[i] : (T)0;
-    const T pu0 = (T)pud[0], pu1 = (T)pud[1], pu2 = (T)pud[2];
-    T tc[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) tc[k] = (T)tcd[k];
-    int gx = base[0];
-    for (int a = 0; a < P; ++a) {
-      int gy = base[1];
-      for (int b = 0; b < P; ++b) {
-        const T xy = Mv[0][a] * Mv[1][b], dxy = dM[0][a] * Mv[1][b], xdy = Mv[0][a] * dM[1][b];
-        // value at z-stencil point c: c0 M(c) + c1 M'(c) + c2 M''(c)
-        T c0 = qi * xy - pu0 * dxy - pu1 * xdy;
-        T c1 = -pu2 * xy;
-        T c2 = (T)0;
-        if constexpr (LVL >= 2) {
-          c0 += tc[0] * d2M[0][a] * Mv[1][b] + tc[1] * Mv[0][a] * d2M[1][b] + tc[3] * dM[0][a] * dM[1][b];
-          c1 += tc[4] * dxy + tc[5] * xdy;
-          c2 = tc[2] * xy;
+    int nc = 0, tot = 0;
+    for (int sx = 0; sx < ns[0]; ++sx)
+      for (int sy = 0; sy < ns[1]; ++sy)
+        for (int sz = 0; sz < ns[2]; ++sz) {
+          const int sb = (src[0][sx] * nb[1] + src[1][sy]) * nb[2] + src[2][sz];
+          s_a0[nc] = start[sb];
+          s_pref[nc] = tot;
+          tot += start[sb + 1] - start[sb];
+          ++nc;
         }
-        double* row = grid + ((long long)gx * K1 + gy) * K2;
-        int gz = base[2];
+    s_pref[nc] = tot;
+    s_nc = nc;
+  }
+  __syncthreads();
+  constexpr int PD = LVL >= 2 ? P : 1;
+  const int ncomb = s_nc, total = s_pref[ncomb];
+  {
+        for (int k = (int)threadIdx.x; k < total; k += blockDim.x) {
+          int c = 0;
+          while (s_pref[c + 1] <= k) ++c;
+          const int i = list[s_a0[c] + (k - s_pref[c])];
+          const double r[3] = {(double)coords[3 * i], (double)coords[3 * i + 1], (double)coords[3 * i + 2]};
+          T Mv[3][P], dM[3][P], d2M[3][PD];
+          int loc[3][P];
+          for (int a = 0; a < 3; ++a) {
+            int base;
+            double t;
+            atom_base(r, g, a, P, base, t);
+            if constexpr (LVL >= 2)
+              spline_weights2<T, P>((T)t, Mv[a], dM[a], d2M[a]);
+            else
+              spline_weights<T, P>((T)t, Mv[a], dM[a]);
+            int gp = base;
 #pragma unroll
-        for (int c = 0; c < P; ++c) {
-          T v = c0 * Mv[2][c] + c1 * dM[2][c];
-          if constexpr (LVL >= 2) v += c2 * d2M[2][c];
-          atomicAdd(&row[gz], (double)v);
-          gz = gz + 1 == K2 ? 0 : gz + 1;
+            for (int c2 = 0; c2 < P; ++c2) {
+              const int l = gp - B0[a];
+              loc[a][c2] = l >= 0 && l < Bn[a] ? l : -1;
+              gp = gp + 1 == g.K[a] ? 0 : gp + 1;
+            }
+          }
+          if constexpr (LVL == 0) {
+            const T qi = ((const T*)mp.q)[i];
+            for (int ka = 0; ka < P; ++ka) {
+              if (loc[0][ka] < 0) continue;
+              const T wa = qi * Mv[0][ka];
+              for (int kb = 0; kb < P; ++kb) {
+                if (loc[1][kb] < 0) continue;
+                const T wab = wa * Mv[1][kb];
+                double* row = s_b + (loc[0][ka] * SB + loc[1][kb]) * SB;
+#pragma unroll
+                for (int kc = 0; kc < P; ++kc)
+                  if (loc[2][kc] >= 0) atomicAdd(&row[loc[2][kc]], (double)(wab * Mv[2][kc]));
+              }
+            }
+          } else {
+            double pud[3], tcd[6];
+            lattice_multipoles<T>(mp, i, g, pud, tcd);
+            const T qi = mp.q ? ((const T*)mp.q)[i] : (T)0;
+            const T pu0 = (T)pud[0], pu1 = (T)pud[1], pu2 = (T)pud[2];
+            T tc[6];
+#pragma unroll
+            for (int k2 = 0; k2 < 6; ++k2) tc[k2] = (T)tcd[k2];
+            for (int ka = 0; ka < P; ++ka) {
+              if (loc[0][ka] < 0) continue;
+              for (int kb = 0; kb < P; ++kb) {
+                if (loc[1][kb] < 0) continue;
+                const T xy = Mv[0][ka] * Mv[1][kb], dxy = dM[0][ka] * Mv[1][kb], xdy = Mv[0][ka] * dM[1][kb];
+                T c0 = qi * xy - pu0 * dxy - pu1 * xdy;
+                T c1 = -pu2 * xy;
+                T c2 = (T)0;
+                if constexpr (LVL >= 2) {
+                  c0 += tc[0] * d2M[0][ka] * Mv[1][kb] + tc[1] * Mv[0][ka] * d2M[1][kb] + tc[3] * dM[0][ka] * dM[1][kb];
+                  c1 += tc[4] * dxy + tc[5] * xdy;
+                  c2 = tc[2] * xy;
+                }
+                double* row = s_b + (loc[0][ka] * SB + loc[1][kb]) * SB;
+#pragma unroll
+                for (int kc = 0; kc < P; ++kc) {
+                  if (loc[2][kc] < 0) continue;
+                  T v = c0 * Mv[2][kc] + c1 * dM[2][kc];
+                  if constexpr (LVL >= 2) v += c2 * d2M[2][kc];
+                  atomicAdd(&row[loc[2][kc]], (double)v);
+                }
+              }
+            }
+          }
         }
-        gy = gy + 1 == K1 ? 0 : gy + 1;
       }
-      gx = gx + 1 == g.K[0] ? 0 : gx + 1;
-    }
+  __syncthreads();
+  for (int k = threadIdx.x; k < SB * SB * SB; k += blockDim.x) {
+    const int lz = k % SB, ly = (k / SB) % SB, lx = k / (SB * SB);
+    if (lx < Bn[0] && ly < Bn[1] && lz < Bn[2])
+      grid[((long long)(B0[0] + lx) * K1 + (B0[1] + ly)) * K2 + (B0[2] + lz)] = s_b[k];
   }
 }
 
@@ -383,11 +504,6 @@ __global__ __launch_bounds__(256) void k_pme_energy_final(const double* __restri
   if (threadIdx.x == 0) out[0] = (T)(s * scale);
 }
 
-__global__ void k_zero_grid(double* p, long long n) {
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  long long s = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += s) p[i] = 0.0;
-}
 
 constexpr int SELF_BLOCKS = 64;
 
@@ -395,11 +511,21 @@ struct PmeWs {
   double* grid;      // K^3 float64 charge grid
   void* work;        // K^3 complex T
   double* parts;     // energy partials
+  int* brick_of;     // natoms: brick of each atom's stencil base
+  int* bcount;       // bricks
+  int* bstart;       // bricks + 1
+  int* bcursor;      // bricks
+  int* blist;        // natoms, by brick
+  int nbricks;
   size_t bytes;
   int nparts;
 };
 
-PmeWs carve(void* base, int dtype, const int K[3]) {
+int brick_count(const int K[3]) {
+  return ((K[0] + SB - 1) / SB) * ((K[1] + SB - 1) / SB) * ((K[2] + SB - 1) / SB);
+}
+
+PmeWs carve(void* base, int dtype, long long natoms, const int K[3]) {
   PmeWs w{};
   size_t n = (size_t)K[0] * K[1] * K[2];
   size_t celem = dtype == TFF_F64 ? 16 : 8;
@@ -415,6 +541,13 @@ PmeWs carve(void* base, int dtype, const int K[3]) {
   w.grid = (double*)take(sizeof(double) * n);
   w.work = (void*)take(celem * n);
   w.parts = (double*)take(sizeof(double) * (size_t)w.nparts);
+  const size_t na = natoms > 0 ? (size_t)natoms : 1;
+  w.nbricks = brick_count(K);
+  w.brick_of = (int*)take(4 * na);
+  w.bcount = (int*)take(4 * (size_t)w.nbricks);
+  w.bstart = (int*)take(4 * ((size_t)w.nbricks + 1));
+  w.bcursor = (int*)take(4 * (size_t)w.nbricks);
+  w.blist = (int*)take(4 * na);
   w.bytes = off;
   return w;
 }
@@ -434,24 +567,42 @@ int spread_level(const Multipoles& mp) { return mp.th ? 2 : (mp.p || !mp.q) ? 1 
   }
 
 template <typename T, int L>
-int spread_level_launch(int order, unsigned nb, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g,
-                        double* grid, hipStream_t st) {
-#define TFF_S(P) hipLaunchKernelGGL((k_spread<T, P, L>), dim3(nb), dim3(256), 0, st, coords, mp, n, g, grid)
+int spread_level_launch(int order, unsigned nbricks, const T* coords, const Multipoles& mp, const PmeGeom& g,
+                        const PmeWs& w, double* grid, hipStream_t st) {
+#define TFF_S(P)                                                                                            \
+  hipLaunchKernelGGL((k_spread<T, P, L>), dim3(nbricks), dim3(256), 0, st, coords, mp, g, (const int*)w.bstart, \
+                     (const int*)w.blist, grid)
   TFF_ORDER_DISPATCH(TFF_S)
 #undef TFF_S
   return TFF_OK;
 }
 
+// bins the atoms by brick, then one workgroup per brick writes it (the whole grid, every call)
 template <typename T>
-int launch_spread(int order, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g, double* grid,
-                  hipStream_t st) {
-  unsigned nb = (unsigned)((n + 255) / 256);
-  if (n == 0) return TFF_OK;
+int launch_spread(int order, const T* coords, const Multipoles& mp, long long n, const PmeGeom& g, const PmeWs& w,
+                  double* grid, hipStream_t st) {
   void* tok = tff_prof_begin("pme_spread", st);
+  const int nbk = w.nbricks;
+  {
+    int zb = (nbk + 255) / 256;
+    zb = zb > 4096 ? 4096 : (zb < 1 ? 1 : zb);
+    hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)zb), dim3(256), 0, st, w.bcount, nbk);
+    hipLaunchKernelGGL(k_zero_ints, dim3((unsigned)zb), dim3(256), 0, st, w.bcursor, nbk);
+  }
+  if (n > 0) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_brick_bin<T>, dim3(nb), dim3(256), 0, st, coords, n, g, w.brick_of, w.bcount);
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, st, (const int*)w.bcount, nbk, w.bstart);
+    hipLaunchKernelGGL(k_brick_scatter, dim3(nb), dim3(256), 0, st, n, (const int*)w.brick_of,
+                       (const int*)w.bstart, w.bcursor, w.blist);
+  } else {
+    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, st, (const int*)w.bcount, nbk, w.bstart);
+  }
   const int lvl = spread_level(mp);
-  int rc = lvl == 0   ? spread_level_launch<T, 0>(order, nb, coords, mp, n, g, grid, st)
-           : lvl == 1 ? spread_level_launch<T, 1>(order, nb, coords, mp, n, g, grid, st)
-                      : spread_level_launch<T, 2>(order, nb, coords, mp, n, g, grid, st);
+  const unsigned nbr = (unsigned)nbk;
+  int rc = lvl == 0   ? spread_level_launch<T, 0>(order, nbr, coords, mp, g, w, grid, st)
+           : lvl == 1 ? spread_level_launch<T, 1>(order, nbr, coords, mp, g, w, grid, st)
+                      : spread_level_launch<T, 2>(order, nbr, coords, mp, g, w, grid, st);
   if (rc) return rc;
   tff_prof_end(tok, st);
   TFF_CHECK_LAUNCH("k_spread");
@@ -513,14 +664,8 @@ int pme_energy(const T* coords, const Multipoles& mp, long long n, const TffBox&
       tff_set_error("PME: grid size %d is not of the form 2^a 3^b 5^c", K[a]);
       return TFF_ERR_UNSUPPORTED;
     }
-  const long long ng = (long long)K[0] * K[1] * K[2];
   PmeGeom g = make_geom(box, K, order, alpha);
-  {
-    long long b = (ng + 255) / 256;
-    if (b > 8192) b = 8192;
-    hipLaunchKernelGGL(k_zero_grid, dim3((unsigned)b), dim3(256), 0, st, w.grid, ng);
-  }
-  int rc = launch_spread<T>(order, coords, mp, n, g, w.grid, st);
+  int rc = launch_spread<T>(order, coords, mp, n, g, w, w.grid, st);
   if (rc) return rc;
   const size_t ce = sizeof(cx<T>);
   ConvArgs cv{};
@@ -575,9 +720,8 @@ int check_geom(const double* box_host, const int K[3], int order) {
 }  // namespace
 
 extern "C" size_t tff_pme_workspace_bytes(int dtype, long long natoms, int K0, int K1, int K2) {
-  (void)natoms;
   int K[3] = {K0, K1, K2};
-  return carve(nullptr, dtype, K).bytes;
+  return carve(nullptr, dtype, natoms, K).bytes;
 }
 
 namespace {
@@ -605,7 +749,7 @@ int pme_energy_entry(const char* name, bool need_q, int dtype, const void* coord
     return TFF_ERR_WORKSPACE;
   }
   TffBox box = tff_make_box(box_host);
-  PmeWs w = carve(workspace, dtype, K);
+  PmeWs w = carve(workspace, dtype, natoms, K);
   if (dtype == TFF_F64)
     return pme_energy<double>((const double*)coords, mp, natoms, box, K, order, alpha, (const double*)xmoduli,
                               (const double*)ymoduli, (const double*)zmoduli, zmoduli_half, scale,
