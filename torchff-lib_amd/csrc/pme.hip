@@ -112,23 +112,47 @@ __device__ __forceinline__ void atom_base(const double* r, const PmeGeom& g, int
   anchor_axis(s, g.K[a], P, base, t);
 }
 
-template <typename T>
-__global__ void k_brick_bin(const T* __restrict__ coords, long long n, PmeGeom g, int* __restrict__ brick_of,
-                            int* __restrict__ count) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double r[3] = {(double)coords[3 * i], (double)coords[3 * i + 1], (double)coords[3 * i + 2]};
-  int bb[3];
-  for (int a = 0; a < 3; ++a) {
-    int base;
-    double t;
-    atom_base(r, g, a, g.order, base, t);
-    bb[a] = base / SB;
+// atomicAdd(&arr[id], 1) for every active lane, one atomic per distinct id in the wave (the lanes
+// of one id are ranked by mbcnt); returns the lane's old value + rank
+__device__ __forceinline__ int wave_agg_add(int* __restrict__ arr, int id, bool active) {
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  unsigned long long pending = __builtin_amdgcn_ballot_w64(active);
+  int result = 0;
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const int lid = __shfl(id, leader, TFF_WAVE);
+    const bool mine = active && id == lid;
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(mine);
+    int old = 0;
+    if (lane == leader) old = atomicAdd(&arr[lid], __builtin_popcountll(m));
+    old = __shfl(old, leader, TFF_WAVE);
+    if (mine)
+      result = old + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    pending &= ~m;
   }
-  const int nb1 = (g.K[1] + SB - 1) / SB, nb2 = (g.K[2] + SB - 1) / SB;
-  const int id = (bb[0] * nb1 + bb[1]) * nb2 + bb[2];
-  brick_of[i] = id;
-  atomicAdd(&count[id], 1);
+  return result;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_brick_bin(const T* __restrict__ coords, long long n, PmeGeom g,
+                                                   int* __restrict__ brick_of, int* __restrict__ count) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  int id = 0;
+  if (active) {
+    const double r[3] = {(double)coords[3 * i], (double)coords[3 * i + 1], (double)coords[3 * i + 2]};
+    int bb[3];
+    for (int a = 0; a < 3; ++a) {
+      int base;
+      double t;
+      atom_base(r, g, a, g.order, base, t);
+      bb[a] = base / SB;
+    }
+    const int nb1 = (g.K[1] + SB - 1) / SB, nb2 = (g.K[2] + SB - 1) / SB;
+    id = (bb[0] * nb1 + bb[1]) * nb2 + bb[2];
+    brick_of[i] = id;
+  }
+  wave_agg_add(count, id, active);
 }
 
 // exclusive scan of count[0..m) into start[0..m], one workgroup of 1024 threads
@@ -155,12 +179,14 @@ __global__ __launch_bounds__(1024) void k_brick_scan(const int* __restrict__ cou
   if (t == 1023) start[m] = s_tot[1023];
 }
 
-__global__ void k_brick_scatter(long long n, const int* __restrict__ brick_of, const int* __restrict__ start,
-                                int* __restrict__ cursor, int* __restrict__ list) {
+__global__ __launch_bounds__(256) void k_brick_scatter(long long n, const int* __restrict__ brick_of,
+                                                       const int* __restrict__ start, int* __restrict__ cursor,
+                                                       int* __restrict__ list) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int id = brick_of[i];
-  list[start[id] + atomicAdd(&cursor[id], 1)] = (int)i;
+  const bool active = i < n;
+  const int id = active ? brick_of[i] : 0;
+  const int pos = wave_agg_add(cursor, id, active);
+  if (active) list[start[id] + pos] = (int)i;
 }
 
 __global__ void k_zero_ints(int* __restrict__ p, int m) {
