@@ -112,25 +112,30 @@ __device__ __forceinline__ void atom_base(const double* r, const PmeGeom& g, int
   anchor_axis(s, g.K[a], P, base, t);
 }
 
-// atomicAdd(&arr[id], 1) for every active lane, one atomic per distinct id in the wave (the lanes
-// of one id are ranked by mbcnt); returns the lane's old value + rank
+// atomicAdd(&arr[id], 1) for every active lane, one atomic per distinct id in the wave: the lanes
+// are grouped by id with ballots (no memory traffic), then the first lane of every group adds the
+// group's size, all groups in one atomic instruction; returns the lane's old value + its rank in
+// its group
 __device__ __forceinline__ int wave_agg_add(int* __restrict__ arr, int id, bool active) {
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   unsigned long long pending = __builtin_amdgcn_ballot_w64(active);
-  int result = 0;
+  int leader_of_mine = lane, rank = 0, size = 0;
   while (pending) {
     const int leader = __builtin_ctzll(pending);
     const int lid = __shfl(id, leader, TFF_WAVE);
     const bool mine = active && id == lid;
     const unsigned long long m = __builtin_amdgcn_ballot_w64(mine);
-    int old = 0;
-    if (lane == leader) old = atomicAdd(&arr[lid], __builtin_popcountll(m));
-    old = __shfl(old, leader, TFF_WAVE);
-    if (mine)
-      result = old + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    if (mine) {
+      leader_of_mine = leader;
+      rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    }
+    if (lane == leader) size = __builtin_popcountll(m);
     pending &= ~m;
   }
-  return result;
+  int old = 0;
+  if (active && lane == leader_of_mine) old = atomicAdd(&arr[id], size);
+  old = __shfl(old, leader_of_mine, TFF_WAVE);
+  return old + rank;
 }
 
 template <typename T>
