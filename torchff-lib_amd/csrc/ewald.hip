@@ -12,8 +12,9 @@
 //
 // Layout: per atom and axis the phases exp(2 pi i h s_a), h = -kmax..kmax (one sincos each, in the
 // input precision); one workgroup per (m_x, m_y) line, the lanes of a wave taking the m_z values
-// and the waves a share of the atoms (fixed-order reduction); one thread per atom for the
-// potential and force, reading S through L1 / L2.  Every sum runs in a fixed order, in float64.
+// and the waves a share of the atoms (fixed-order reduction), G S stored per vector; one wave per
+// atom for the potential and force, the lanes striding over the vectors (reading G S through
+// L1 / L2) and a wave sum at the end.  Every sum runs in a fixed order, in float64.
 
 #include "tff_common.h"
 #include "torchff_hip.h"
@@ -118,49 +119,58 @@ __global__ __launch_bounds__(EW_WAVES * TFF_WAVE) void k_ewald_structure(const T
   if (lane == 0) eline[line] = e;
 }
 
-// potential (incl. the self term) and force of one atom per thread from G S; energy by one thread
+// potential (incl. the self term) and force of one atom per wave: the lanes stride over the
+// half-space vectors (G S from the structure pass), then a wave sum; the energy by block 0
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_atoms(const T* __restrict__ q, long long n, int kmax, TffBox box,
-                                                     double alpha, int nlines, const CX<T>* __restrict__ tab,
-                                                     const double* __restrict__ S,
-                                                     const double* __restrict__ eline, double* __restrict__ energy,
-                                                     T* __restrict__ pot, T* __restrict__ forces) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(TFF_WAVE) void k_ewald_atoms(const T* __restrict__ q, long long n, int kmax, TffBox box,
+                                                          double alpha, int nlines, const CX<T>* __restrict__ tab,
+                                                          const double* __restrict__ S,
+                                                          const double* __restrict__ eline,
+                                                          double* __restrict__ energy, T* __restrict__ pot,
+                                                          T* __restrict__ forces) {
+  const long long i = blockIdx.x;
+  const int lane = threadIdx.x;
   const int W = 2 * kmax + 1;
   const double V = box.det;
-  if (blockIdx.x == 0 && threadIdx.x < TFF_WAVE && energy) {
-    // energy (first wave): line partials and q^2, each lane a fixed stride, then the wave sum
+  if (blockIdx.x == 0 && energy) {
+    // energy: line partials and q^2, each lane a fixed stride, then the wave sum
     double e = 0, q2 = 0;
-    for (int l = threadIdx.x; l < nlines; l += TFF_WAVE) e += eline[l];
-    for (long long j = threadIdx.x; j < n; j += TFF_WAVE) q2 += (double)q[j] * (double)q[j];
+    for (int l = lane; l < nlines; l += TFF_WAVE) e += eline[l];
+    for (long long j = lane; j < n; j += TFF_WAVE) q2 += (double)q[j] * (double)q[j];
     e = tff_wave_sum(e);
     q2 = tff_wave_sum(q2);
-    if (threadIdx.x == 0) *energy = e / (M_PI * V) - alpha / sqrt(M_PI) * q2;
+    if (lane == 0) *energy = e / (M_PI * V) - alpha / sqrt(M_PI) * q2;
   }
   if (i >= n || (!pot && !forces)) return;
+  const CX<T>* tx = tab + (0 * n + i) * W;
+  const CX<T>* ty = tab + (1 * n + i) * W;
+  const CX<T>* tz = tab + (2 * n + i) * W;
   double ph = 0, fx = 0, fy = 0, fz = 0;
-  for (int line = 0; line < nlines; ++line) {
+  const int total = nlines * W;
+  for (int kk = lane; kk < total; kk += TFF_WAVE) {
+    const int line = kk / W, h = kk - line * W;
     int mx, my;
     line_of(line, kmax, mx, my);
-    const CX<T> tx = tab[(0 * n + i) * W + mx + kmax];
-    const CX<T> ty = tab[(1 * n + i) * W + my + kmax];
-    const double ax = (double)tx.x * ty.x - (double)tx.y * ty.y, ay = (double)tx.x * ty.y + (double)tx.y * ty.x;
-    const CX<T>* tz = tab + (2 * n + i) * W;
-    for (int h = 0; h < W; ++h) {
-      const int mz = h - kmax;
-      if (mx == 0 && my == 0 && mz <= 0) continue;
-      const double c = ax * tz[h].x - ay * tz[h].y, s = ax * tz[h].y + ay * tz[h].x;
-      const double kx = box.inv[0] * mx + box.inv[1] * my + box.inv[2] * mz;
-      const double ky = box.inv[3] * mx + box.inv[4] * my + box.inv[5] * mz;
-      const double kz = box.inv[6] * mx + box.inv[7] * my + box.inv[8] * mz;
-      const double Sr = S[2 * ((size_t)line * W + h)], Si = S[2 * ((size_t)line * W + h) + 1];  // G S
-      ph += Sr * c + Si * s;
-      const double fi = Sr * s - Si * c;
-      fx += fi * kx;
-      fy += fi * ky;
-      fz += fi * kz;
-    }
+    const int mz = h - kmax;
+    if (mx == 0 && my == 0 && mz <= 0) continue;
+    const CX<T> ax_ = tx[mx + kmax], ay_ = ty[my + kmax], az_ = tz[h];
+    const double ax = (double)ax_.x * ay_.x - (double)ax_.y * ay_.y, ay = (double)ax_.x * ay_.y + (double)ax_.y * ay_.x;
+    const double c = ax * az_.x - ay * az_.y, s = ax * az_.y + ay * az_.x;
+    const double kx = box.inv[0] * mx + box.inv[1] * my + box.inv[2] * mz;
+    const double ky = box.inv[3] * mx + box.inv[4] * my + box.inv[5] * mz;
+    const double kz = box.inv[6] * mx + box.inv[7] * my + box.inv[8] * mz;
+    const double Sr = S[2 * (size_t)kk], Si = S[2 * (size_t)kk + 1];  // G S
+    ph += Sr * c + Si * s;
+    const double fi = Sr * s - Si * c;
+    fx += fi * kx;
+    fy += fi * ky;
+    fz += fi * kz;
   }
+  ph = tff_wave_sum(ph);
+  fx = tff_wave_sum(fx);
+  fy = tff_wave_sum(fy);
+  fz = tff_wave_sum(fz);
+  if (lane != 0) return;
   const double qi = (double)q[i];
   if (pot) pot[i] = (T)(2.0 / (M_PI * V) * ph - 2.0 * alpha / sqrt(M_PI) * qi);
   if (forces) {
@@ -270,90 +280,117 @@ __global__ __launch_bounds__(EW_WAVES * TFF_WAVE) void k_ewald_structure_mp(EwMp
   if (lane == 0) eline[line] = e;
 }
 
+// one atom's multipoles in registers
+struct EwAtom {
+  double q, p[3], th[9];
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_atoms_mp(EwMp src, EwMp con, long long n, int kmax, TffBox box,
-                                                        double alpha, int nlines, const CX<T>* __restrict__ tab,
-                                                        const double* __restrict__ S,
-                                                        const double* __restrict__ eline, double* __restrict__ energy,
-                                                        T* __restrict__ pot, T* __restrict__ field,
-                                                        T* __restrict__ fgrad, T* __restrict__ forces) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ EwAtom ew_atom(const EwMp& m, long long j) {
+  EwAtom a{};
+  if (m.q) a.q = ew_load<T>(m.q, j);
+  if (m.p)
+    for (int k = 0; k < 3; ++k) a.p[k] = ew_load<T>(m.p, 3 * j + k);
+  if (m.th)
+    for (int k = 0; k < 9; ++k) a.th[k] = ew_load<T>(m.th, 9 * j + k);
+  return a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TFF_WAVE) void k_ewald_atoms_mp(EwMp src, EwMp con, long long n, int kmax, TffBox box,
+                                                             double alpha, int nlines, const CX<T>* __restrict__ tab,
+                                                             const double* __restrict__ S,
+                                                             const double* __restrict__ eline,
+                                                             double* __restrict__ energy, T* __restrict__ pot,
+                                                             T* __restrict__ field, T* __restrict__ fgrad,
+                                                             T* __restrict__ forces) {
+  const long long i = blockIdx.x;
+  const int lane = threadIdx.x;
   const int W = 2 * kmax + 1;
   const double V = box.det;
   const double rpi = 1.0 / sqrt(M_PI), a2 = alpha * alpha;
-  if (blockIdx.x == 0 && threadIdx.x < TFF_WAVE && energy) {
+  if (blockIdx.x == 0 && energy) {
     double e = 0, self = 0;
-    for (int l = threadIdx.x; l < nlines; l += TFF_WAVE) e += eline[l];
-    for (long long j = threadIdx.x; j < n; j += TFF_WAVE) {
-      if (src.q) {
-        const double q = ew_load<T>(src.q, j);
-        self += q * q;
-      }
-      if (src.p)
-        for (int k = 0; k < 3; ++k) {
-          const double v = ew_load<T>(src.p, 3 * j + k);
-          self += 2.0 * a2 / 3.0 * v * v;
-        }
-      if (src.th)
-        for (int k = 0; k < 9; ++k) {
-          const double v = ew_load<T>(src.th, 9 * j + k);
-          self += 8.0 * a2 * a2 / 45.0 * v * v;
-        }
+    for (int l = lane; l < nlines; l += TFF_WAVE) e += eline[l];
+    for (long long j = lane; j < n; j += TFF_WAVE) {
+      const EwAtom a = ew_atom<T>(src, j);
+      self += a.q * a.q;
+      for (int k = 0; k < 3; ++k) self += 2.0 * a2 / 3.0 * a.p[k] * a.p[k];
+      for (int k = 0; k < 9; ++k) self += 8.0 * a2 * a2 / 45.0 * a.th[k] * a.th[k];
     }
     e = tff_wave_sum(e);
     self = tff_wave_sum(self);
-    if (threadIdx.x == 0) *energy = e / (M_PI * V) - alpha * rpi * self;
+    if (lane == 0) *energy = e / (M_PI * V) - alpha * rpi * self;
   }
   if (i >= n || (!pot && !field && !fgrad && !forces)) return;
+  const EwAtom X = ew_atom<T>(con, i);  // the contracted set, loaded once
+  const bool xp = con.p != nullptr, xt = con.th != nullptr;
+  const CX<T>* tx = tab + (0 * n + i) * W;
+  const CX<T>* ty = tab + (1 * n + i) * W;
+  const CX<T>* tz = tab + (2 * n + i) * W;
   double sar = 0, fa[3] = {0, 0, 0}, eg[6] = {0, 0, 0, 0, 0, 0}, fk[3] = {0, 0, 0};
-  for (int line = 0; line < nlines; ++line) {
+  const int total = nlines * W;
+  for (int kk = lane; kk < total; kk += TFF_WAVE) {
+    const int line = kk / W, h = kk - line * W;
     int mx, my;
     line_of(line, kmax, mx, my);
-    const CX<T> tx = tab[(0 * n + i) * W + mx + kmax];
-    const CX<T> ty = tab[(1 * n + i) * W + my + kmax];
-    const double ax = (double)tx.x * ty.x - (double)tx.y * ty.y, ay = (double)tx.x * ty.y + (double)tx.y * ty.x;
-    const CX<T>* tz = tab + (2 * n + i) * W;
-    for (int h = 0; h < W; ++h) {
-      const int mz = h - kmax;
-      if (mx == 0 && my == 0 && mz <= 0) continue;
-      const double c = ax * tz[h].x - ay * tz[h].y, s = ax * tz[h].y + ay * tz[h].x;
-      double kx, ky, kz;
-      ew_k(box, mx, my, mz, kx, ky, kz);
-      const double Sr = S[2 * ((size_t)line * W + h)], Si = S[2 * ((size_t)line * W + h) + 1];  // G S
-      const double Ar = Sr * c + Si * s, Ai = Sr * s - Si * c;
-      sar += Ar;
-      fa[0] += kx * Ai;
-      fa[1] += ky * Ai;
-      fa[2] += kz * Ai;
-      if (fgrad) {
-        eg[0] += Ar * kx * kx;
-        eg[1] += Ar * ky * ky;
-        eg[2] += Ar * kz * kz;
-        eg[3] += Ar * kx * ky;
-        eg[4] += Ar * kx * kz;
-        eg[5] += Ar * ky * kz;
+    const int mz = h - kmax;
+    if (mx == 0 && my == 0 && mz <= 0) continue;
+    const CX<T> ax_ = tx[mx + kmax], ay_ = ty[my + kmax], az_ = tz[h];
+    const double ax = (double)ax_.x * ay_.x - (double)ax_.y * ay_.y, ay = (double)ax_.x * ay_.y + (double)ax_.y * ay_.x;
+    const double c = ax * az_.x - ay * az_.y, s = ax * az_.y + ay * az_.x;
+    double kx, ky, kz;
+    ew_k(box, mx, my, mz, kx, ky, kz);
+    const double Sr = S[2 * (size_t)kk], Si = S[2 * (size_t)kk + 1];  // G S
+    const double Ar = Sr * c + Si * s, Ai = Sr * s - Si * c;
+    sar += Ar;
+    fa[0] += kx * Ai;
+    fa[1] += ky * Ai;
+    fa[2] += kz * Ai;
+    if (fgrad) {
+      eg[0] += Ar * kx * kx;
+      eg[1] += Ar * ky * ky;
+      eg[2] += Ar * kz * kz;
+      eg[3] += Ar * kx * ky;
+      eg[4] += Ar * kx * kz;
+      eg[5] += Ar * ky * kz;
+    }
+    if (forces) {
+      double lr = X.q, li = 0.0;
+      if (xp) li = 2.0 * M_PI * (kx * X.p[0] + ky * X.p[1] + kz * X.p[2]);
+      if (xt) {
+        const double kv[3] = {kx, ky, kz};
+        double acc = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+          for (int y = 0; y < 3; ++y) acc += kv[x] * X.th[3 * x + y] * kv[y];
+        lr -= 4.0 * M_PI * M_PI / 3.0 * acc;
       }
-      if (forces) {
-        double lr, li;
-        ew_L<T>(con, i, kx, ky, kz, lr, li);
-        const double im = lr * Ai + li * Ar;
-        fk[0] += kx * im;
-        fk[1] += ky * im;
-        fk[2] += kz * im;
-      }
+      const double im = lr * Ai + li * Ar;
+      fk[0] += kx * im;
+      fk[1] += ky * im;
+      fk[2] += kz * im;
     }
   }
-  if (pot) pot[i] = (T)(2.0 / (M_PI * V) * sar - (src.q ? 2.0 * alpha * rpi * ew_load<T>(src.q, i) : 0.0));
+  sar = tff_wave_sum(sar);
+  for (int k = 0; k < 3; ++k) {
+    fa[k] = tff_wave_sum(fa[k]);
+    fk[k] = tff_wave_sum(fk[k]);
+  }
+  if (fgrad)
+    for (int k = 0; k < 6; ++k) eg[k] = tff_wave_sum(eg[k]);
+  if (lane != 0) return;
+  const EwAtom A = ew_atom<T>(src, i);
+  if (pot) pot[i] = (T)(2.0 / (M_PI * V) * sar - 2.0 * alpha * rpi * A.q);
   if (field) {
     const double cf = 4.0 * alpha * a2 * rpi / 3.0;
-    for (int x = 0; x < 3; ++x)
-      field[3 * i + x] = (T)(4.0 / V * fa[x] + (src.p ? cf * ew_load<T>(src.p, 3 * i + x) : 0.0));
+    for (int x = 0; x < 3; ++x) field[3 * i + x] = (T)(4.0 / V * fa[x] + cf * A.p[x]);
   }
   if (fgrad) {
     const double cg = 16.0 * alpha * a2 * a2 * rpi / 15.0;
     const int map[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
-    for (int k = 0; k < 9; ++k)
-      fgrad[9 * i + k] = (T)(8.0 * M_PI / V * eg[map[k]] + (src.th ? cg * ew_load<T>(src.th, 9 * i + k) : 0.0));
+    for (int k = 0; k < 9; ++k) fgrad[9 * i + k] = (T)(8.0 * M_PI / V * eg[map[k]] + cg * A.th[k]);
   }
   if (forces)
     for (int x = 0; x < 3; ++x) forces[3 * i + x] = (T)(4.0 / V * fk[x]);
@@ -395,8 +432,8 @@ int run_ewald(const T* coords, const T* q, long long n, const TffBox& box, int k
                      (CX<T>*)w.tab);
   hipLaunchKernelGGL(k_ewald_structure<T>, dim3(nlines), dim3(EW_WAVES * TFF_WAVE), 0, st, q, n, kmax, box, alpha,
                      (const CX<T>*)w.tab, w.S, w.eline);
-  const unsigned g3 = (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
-  hipLaunchKernelGGL(k_ewald_atoms<T>, dim3(g3), dim3(256), 0, st, q, n, kmax, box, alpha, nlines,
+  const unsigned g3 = (unsigned)(n > 0 ? n : 1);  // one wave per atom
+  hipLaunchKernelGGL(k_ewald_atoms<T>, dim3(g3), dim3(TFF_WAVE), 0, st, q, n, kmax, box, alpha, nlines,
                      (const CX<T>*)w.tab, (const double*)w.S, (const double*)w.eline, energy, pot, forces);
   TFF_CHECK_LAUNCH("ewald");
   return TFF_OK;
@@ -414,8 +451,8 @@ int run_ewald_mp(const T* coords, const EwMp& src, const EwMp& con, long long n,
                      (CX<T>*)w.tab);
   hipLaunchKernelGGL(k_ewald_structure_mp<T>, dim3(nlines), dim3(EW_WAVES * TFF_WAVE), 0, st, src, n, kmax, box,
                      alpha, (const CX<T>*)w.tab, w.S, w.eline);
-  const unsigned g3 = (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1);
-  hipLaunchKernelGGL(k_ewald_atoms_mp<T>, dim3(g3), dim3(256), 0, st, src, con, n, kmax, box, alpha, nlines,
+  const unsigned g3 = (unsigned)(n > 0 ? n : 1);  // one wave per atom
+  hipLaunchKernelGGL(k_ewald_atoms_mp<T>, dim3(g3), dim3(TFF_WAVE), 0, st, src, con, n, kmax, box, alpha, nlines,
                      (const CX<T>*)w.tab, (const double*)w.S, (const double*)w.eline, energy, pot, field, fgrad,
                      forces);
   TFF_CHECK_LAUNCH("ewald multipole");
