@@ -1858,6 +1858,12 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   }
   TC* phi = (TC*)w.grid;  // the real potential grid reuses the charge grid's memory after the z pass
   run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts, "fused_fft", sp);
+  {
+    void* tok = tff_prof_begin("fused_gather", sp);
+    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, phi, coul, false, status, sp);
+    tff_prof_end(tok, sp);
+    if (rc) return rc;
+  }
   // 4. cluster pairs
   {
     void* tok = tff_prof_begin("fused_cluster_pairs", st);
@@ -1892,16 +1898,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
                          w.eacc);
     tff_prof_end(tok, st);
   }
-  // 6. the reciprocal-space forces and the outputs: the gather runs on the caller's stream after the
-  // pair kernel (the spread and FFT have normally finished beside the pair search and pair kernel),
-  // so that it has the whole device instead of sharing it with the pair kernel
+  // 6. outputs, after the reciprocal-space work
   aux_stream_join(st, sp);
-  {
-    void* tok = tff_prof_begin("fused_gather", st);
-    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, phi, coul, false, status, st);
-    tff_prof_end(tok, st);
-    if (rc) return rc;
-  }
   hipLaunchKernelGGL(k_finalize<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, w.facc_pme, forces, w.eacc,
                      w.kparts, w.nk, w.qparts, w.nq, coul, alpha, energies);
   TFF_CHECK_LAUNCH("fused energy+forces");
