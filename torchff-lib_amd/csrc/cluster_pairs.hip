@@ -533,29 +533,36 @@ __global__ __launch_bounds__(CPB) void k_cp_nonbonded(const T* __restrict__ coor
         fi[1] += (double)(dedr_over_r * dy);
         fi[2] += (double)(dedr_over_r * dz);
       }
-      // the column atom's share: reduce over the half-wave, one atomic per column
-      double cx = ok ? -(double)(dedr_over_r * dx) : 0.0, cy = ok ? -(double)(dedr_over_r * dy) : 0.0,
-             cz = ok ? -(double)(dedr_over_r * dz) : 0.0;
-      double cq = ok ? (double)gqc : 0.0, cs = ok ? (double)gsc : 0.0, ce = ok ? (double)gec : 0.0;
+      // the column atom's share: summed over the half-wave in the input precision (only the
+      // requested gradients), one atomic per column
       const unsigned any = half_ballot(ok);
       if (any) {
-        for (int o = 16; o > 0; o >>= 1) {
-          cx += __shfl_xor(cx, o, 32);
-          cy += __shfl_xor(cy, o, 32);
-          cz += __shfl_xor(cz, o, 32);
-          cq += __shfl_xor(cq, o, 32);
-          cs += __shfl_xor(cs, o, 32);
-          ce += __shfl_xor(ce, o, 32);
-        }
-        if (l == c) {
-          if (g_xyz) {
-            atomicAdd(&g_xyz[3 * jc], cx);
-            atomicAdd(&g_xyz[3 * jc + 1], cy);
-            atomicAdd(&g_xyz[3 * jc + 2], cz);
+        if (g_xyz) {
+          T cx = ok ? -(dedr_over_r * dx) : (T)0, cy = ok ? -(dedr_over_r * dy) : (T)0,
+            cz = ok ? -(dedr_over_r * dz) : (T)0;
+          for (int o = 16; o > 0; o >>= 1) {
+            cx += __shfl_xor(cx, o, 32);
+            cy += __shfl_xor(cy, o, 32);
+            cz += __shfl_xor(cz, o, 32);
           }
-          if (g_q) atomicAdd(&g_q[jc], cq);
-          if (g_sig) atomicAdd(&g_sig[jc], cs);
-          if (g_eps) atomicAdd(&g_eps[jc], ce);
+          if (l == c) {
+            atomicAdd(&g_xyz[3 * jc], (double)cx);
+            atomicAdd(&g_xyz[3 * jc + 1], (double)cy);
+            atomicAdd(&g_xyz[3 * jc + 2], (double)cz);
+          }
+        }
+        if (g_q || g_sig || g_eps) {
+          T cq = ok ? gqc : (T)0, cs = ok ? gsc : (T)0, ce = ok ? gec : (T)0;
+          for (int o = 16; o > 0; o >>= 1) {
+            cq += __shfl_xor(cq, o, 32);
+            cs += __shfl_xor(cs, o, 32);
+            ce += __shfl_xor(ce, o, 32);
+          }
+          if (l == c) {
+            if (g_q) atomicAdd(&g_q[jc], (double)cq);
+            if (g_sig) atomicAdd(&g_sig[jc], (double)cs);
+            if (g_eps) atomicAdd(&g_eps[jc], (double)ce);
+          }
         }
       }
     }
