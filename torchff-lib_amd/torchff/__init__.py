@@ -23,3 +23,4 @@ from .vdw import Vdw, compute_lennard_jones_energy  # noqa: F401
 from .pme import PME  # noqa: F401
 from .ewald import Ewald, ewald_long_range, ewald_long_range_with_fields  # noqa: F401
 from . import bsplines  # noqa: F401
+from .md import BatchedSimModel  # noqa: F401
