@@ -277,6 +277,34 @@ def widened_bench(args, w, dev):
         m = torchff.Ewald(w9["alpha"], 10, rank, return_fields=False)
         sec, _ = run(m, *inputs(w9, torch.float64, rank, 10 + rank), 5, None)
         res[f"ewald_rank{rank}_f64_water900_kmax10"] = {"ms": sec * 1e3, "evals_per_s": 1.0 / sec}
+    # the torch-sim style consumer: 16 water boxes of 20-300 waters in one state per call, against
+    # the same boxes one call each (mixed precision serving engine)
+    from types import SimpleNamespace
+    from torchff.md import BatchedSimModel, NM_PER_ANGSTROM
+    from torchff.serving import ServingEngine
+    from torchff.workloads import box_length_for_density, water_box
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(20), np.log(300), 16))]
+    ws = [water_box(nw, box_length_for_density(nw), seed=100 + k) for k, nw in enumerate(sizes)]
+    eng = ServingEngine(slots=2, max_systems=16, max_atoms=16384, max_grid_points=1 << 18, max_k=32,
+                        precision="mixed", device=dev)
+
+    def state_of(sel):
+        P = np.concatenate([ws[k]["coords"] for k in sel]) / NM_PER_ANGSTROM
+        idx = np.concatenate([np.full(ws[k]["natoms"], j) for j, k in enumerate(sel)])
+        cells = np.stack([np.asarray(ws[k]["box"]).T / NM_PER_ANGSTROM for k in sel])
+        par = [np.concatenate([np.asarray(ws[k][f]) for k in sel]) for f in ("charges", "sigma", "epsilon")]
+        mol = np.concatenate([np.arange(ws[k]["natoms"]) // 3 for k in sel])
+        st = SimpleNamespace(positions=torch.tensor(P, device=dev), cell=torch.tensor(cells, device=dev),
+                             system_idx=torch.tensor(idx, device=dev))
+        return st, BatchedSimModel(eng, *par, mol)
+
+    st_all, m_all = state_of(list(range(16)))
+    singles = [state_of([k]) for k in range(16)]
+    sec_all, _ = time_steps(lambda: m_all(st_all), 20, 3)
+    sec_one, _ = time_steps(lambda: [m(st) for st, m in singles], 5, 1)
+    res["md_batched_16_systems"] = {"ms": sec_all * 1e3, "systems": 16, "waters": sizes,
+                                    "one_call_per_system_ms": sec_one * 1e3}
     head = res["pme_rank2_f32"]
     print(json.dumps({
         "metric": f"PME energy + gradient evals/s, point charges + dipoles + quadrupoles, {w['natoms'] // 1000}k "
@@ -285,7 +313,8 @@ def widened_bench(args, w, dev):
         "warmup": 2, "ms_per_step": head["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload} coordinates and charges, random dipoles and traceless quadrupoles, "
-                               f"PME {w['K']}^3 order {w['order']}; direct Ewald on water900 with kmax 10",
+                               f"PME {w['K']}^3 order {w['order']}; direct Ewald on water900 with kmax 10; "
+                               "16 water boxes through the torch-sim style model",
                    "path": "torchff.PME(rank) / torchff.Ewald(rank) drop-in ops, autograd backward"},
         "results": res}))
 
