@@ -11,8 +11,8 @@
 //   F_i = -dE/dr_i = 4 q_i / V sum_k G k (Re S sin theta_i - Im S cos theta_i),  theta_i = 2 pi m . s_i.
 //
 // Layout: per atom and axis the phases exp(2 pi i h s_a), h = -kmax..kmax (one sincos each, in the
-// input precision); one workgroup per (m_x, m_y) line, the lanes of a wave taking the m_z values
-// and the waves a share of the atoms (fixed-order reduction), G S stored per vector; one wave per
+// input precision); one workgroup of 16 waves per (m_x, m_y) line, the lanes of a wave taking the
+// m_z values and the waves a share of the atoms (fixed-order reduction), G S stored per vector; one wave per
 // atom for the potential and force, the lanes striding over the vectors (reading G S through
 // L1 / L2) and a wave sum at the end.  Every sum runs in a fixed order, in float64.
 
@@ -21,7 +21,7 @@
 
 namespace {
 
-constexpr int EW_WAVES = 4;  // waves per line workgroup
+constexpr int EW_WAVES = 16;  // waves per line workgroup
 constexpr int EW_MAXK = 31;  // 2 kmax + 1 lanes
 
 template <typename T> struct CX { T x, y; };
