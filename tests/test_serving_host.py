@@ -93,6 +93,19 @@ def test_table_rejects_bad_systems(lib):
     assert list(_totals(lib, tp)) == [0] * 8
 
 
+@pytest.mark.parametrize("fill", [0xFF, 0x7F, 0x01])
+def test_table_refuses_an_unreset_head(lib, fill):
+    # a table whose head was never reset (pinned memory comes uninitialised) is refused,
+    # not indexed by its garbage system count
+    buf, tp = _table(lib, 4)
+    buf[:] = fill
+    caps = (4, 1 << 14, 32, 1 << 16)
+    assert _add(lib, tp, caps, 0, 60, 0.843, 9) not in (0, L.SERVE_FULL)
+    assert "reset" in lib.tff_last_error().decode()
+    lib.tff_serve_table_reset(tp)
+    assert _add(lib, tp, caps, 0, 60, 0.843, 9) == 0
+
+
 def test_workspace_lds_limit(lib):
     # a slab transform (two power-of-two line sets of edge + 1 values) must fit 64 KiB of LDS:
     # max_k <= 62 (float32), 32 (float64)

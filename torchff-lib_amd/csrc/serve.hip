@@ -713,6 +713,12 @@ extern "C" int tff_serve_table_add(void* table_host, int compute_dtype, int max_
     return TFF_ERR_ARG;
   }
   BatchHead* h = (BatchHead*)table_host;
+  if (h->nsys < 0 || h->nsys > max_systems || h->natoms < 0 || h->ntask < 0 || h->nwork < 0 || h->nslab < 0 ||
+      h->ntile[0] < 0 || h->ntile[1] < 0 || h->ntile[2] < 0 || h->atom_end < 0 || h->atom_end > max_atoms) {
+    tff_set_error("serve: table head holds %d systems / %d atom records for capacity %d / %d "
+                  "(a table must be reset before its first system)", h->nsys, h->atom_end, max_systems, max_atoms);
+    return TFF_ERR_ARG;
+  }
   SysDesc* recs = (SysDesc*)((char*)table_host + HEAD_BYTES);
   SysDesc d{};
   for (int a = 0; a < 3; ++a) {

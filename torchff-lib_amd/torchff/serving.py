@@ -140,8 +140,8 @@ class _Slot:
         self.d_out = torch.empty(self.nout, dtype=torch.float64, device=dev)
         self.h_out = torch.empty(self.nout, dtype=torch.float64, pin_memory=True)
         self.n_out = self.h_out.numpy()
-        self.requests = []
-        self.natoms = 0
+        # the pinned table comes uninitialised (possibly a reused block): clear its head now
+        self.reset()
         self.event = torch.cuda.Event()
         self.graph = None
         self.bytes = nbytes + self.nout * 8
