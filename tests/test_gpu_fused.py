@@ -88,6 +88,24 @@ def test_fused_small_odd_box():
 
 
 @pytest.mark.parametrize("precision,te,tf", [("fp64", 1e-9, 1e-9), ("mixed", 1e-7, 1e-4)])
+def test_fused_shared_z_planes(precision, te, tf):
+    # one 30 x 30 layer of waters whose oxygens all sit at exactly the same z (no jitter), in a
+    # 9.3 x 9.3 x 6.0 nm box: about 36 oxygens of every x-y column fall into one z bucket of the
+    # column sort, above its 32-key bucket limit, so the columns take the bitonic fallback
+    w = water_box(900, 9.3, n_lat=30, seed=41, jitter=0.0)
+    w["coords"] = np.ascontiguousarray(w["coords"][:, [1, 2, 0]])
+    w["box"] = np.diag([9.3, 9.3, 6.0])
+    assert np.sum(w["coords"][0::3, 2] == w["coords"][0, 2]) == 900
+    w.update(cutoff=0.9, skin=0.0, K=64, order=5, alpha=ewald_alpha(0.9))
+    w["name"] = "water_layer2700"
+    E, F, terms, _ = composite.energy_forces(w)
+    Ed, Fd, _ = _run(w, precision)
+    scale = sum(abs(v) for v in terms.values())
+    assert abs(Ed - E) < te * scale, (Ed, E)
+    assert _rms_rel(Fd, F) < tf
+
+
+@pytest.mark.parametrize("precision,te,tf", [("fp64", 1e-9, 1e-9), ("mixed", 1e-7, 1e-4)])
 def test_fused_tiny_grid(precision, te, tf):
     # 64 waters at 1 g/cm3 (L ~ 1.24 nm), rc 0.6, a 10^3 grid: every axis is shorter than a
     # spreading brick plus a stencil, so the stencil wraps into both ends of a brick

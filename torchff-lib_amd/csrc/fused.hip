@@ -171,34 +171,107 @@ __global__ __launch_bounds__(FB) void k_scatter_atoms(long long n, const int* __
   sorted_orig[col_start[col_of[i]] + off_in_col[i]] = (int)i;
 }
 
-// One workgroup per column: bitonic sort of the keys (float bits of the wrapped z << 32 | atom)
-// in LDS.  Columns above COL_SORT_MAX atoms are left in scatter order (flagged by the scan).
+// One workgroup per column sorts the keys (float bits of the wrapped z << 32 | atom) in LDS.  The
+// keys are distinct (the atom index), so the sorted order is unique.  Bucket sort: the keys are
+// counted into up to COL_BUCKETS z buckets (the bucket is monotone in the key), the counts
+// scanned, the keys placed by bucket, and every key written to its bucket start plus its rank
+// among the keys of its bucket; a column whose largest bucket holds more than BUCKET_RANK_MAX
+// keys (atoms on shared z planes) is sorted by a bitonic network over the placed keys instead.
+// Columns above COL_SORT_MAX atoms are left in scatter order (flagged by the scan).
+constexpr int COL_BUCKETS = 2048;
+constexpr int BUCKET_RANK_MAX = 32;
+constexpr int SORT_THREADS = 256;
+
 template <typename TI>
-__global__ __launch_bounds__(256) void k_sort_columns(const TI* __restrict__ coords, Geo g,
-                                                      const int* __restrict__ col_start,
-                                                      int* __restrict__ sorted_orig) {
+__device__ __forceinline__ unsigned long long column_key(const TI* __restrict__ coords, int i, const Geo& g) {
+  double w[3];
+  wrap_coords(coords, i, g, w);
+  return ((unsigned long long)__float_as_uint((float)w[2]) << 32) | (unsigned)i;
+}
+// z >= 0, so the float bits order as the values and the product is monotone in the key
+__device__ __forceinline__ int column_bucket(unsigned long long key, float scale, int nb) {
+  const int b = (int)(__uint_as_float((unsigned)(key >> 32)) * scale);
+  return b < 0 ? 0 : (b < nb - 1 ? b : nb - 1);
+}
+
+template <typename TI>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_columns(const TI* __restrict__ coords, Geo g,
+                                                               const int* __restrict__ col_start,
+                                                               int* __restrict__ sorted_orig) {
+  static_assert(COL_BUCKETS <= 8 * SORT_THREADS, "column sort: eight buckets per thread at most");
   __shared__ unsigned long long s_k[COL_SORT_MAX];
+  __shared__ int s_c[COL_BUCKETS];
+  __shared__ int s_tot[SORT_THREADS / TFF_WAVE], s_max[SORT_THREADS / TFF_WAVE];
   const int col = blockIdx.x;
   const int cs = col_start[col], m = col_start[col + 1] - cs;
   if (m <= 1 || m > COL_SORT_MAX) return;
-  int np = 1;
-  while (np < m) np <<= 1;
-  for (int k = threadIdx.x; k < np; k += blockDim.x) {
-    unsigned long long key = ~0ull;
-    if (k < m) {
-      const int i = sorted_orig[cs + k];
-      double w[3];
-      wrap_coords(coords, i, g, w);
-      key = ((unsigned long long)__float_as_uint((float)w[2]) << 32) | (unsigned)i;
-    }
-    s_k[k] = key;
+  const int t = threadIdx.x, lane = t & (TFF_WAVE - 1), wv = t / TFF_WAVE;
+  const int nb = m / 2 < 1 ? 1 : (m / 2 < COL_BUCKETS ? m / 2 : COL_BUCKETS);
+  const float scale = (float)nb / (float)g.L[2];
+  for (int b = t; b < nb; b += SORT_THREADS) s_c[b] = 0;
+  __syncthreads();
+  for (int k = t; k < m; k += SORT_THREADS)
+    atomicAdd(&s_c[column_bucket(column_key(coords, sorted_orig[cs + k], g), scale, nb)], 1);
+  __syncthreads();
+  // exclusive scan of the bucket sizes (thread t owns buckets [b0, b1)) and the largest bucket
+  const int per = (nb + SORT_THREADS - 1) / SORT_THREADS;
+  const int b0 = t * per < nb ? t * per : nb, b1 = b0 + per < nb ? b0 + per : nb;
+  int sum = 0, mx = 0;
+  for (int b = b0; b < b1; ++b) {
+    const int c = s_c[b];
+    sum += c;
+    mx = c > mx ? c : mx;
+  }
+  int inc = sum;
+  for (int o = 1; o < TFF_WAVE; o <<= 1) {
+    const int v = __shfl_up(inc, o, TFF_WAVE);
+    if (lane >= o) inc += v;
+  }
+  for (int o = TFF_WAVE / 2; o > 0; o >>= 1) {
+    const int v = __shfl_xor(mx, o, TFF_WAVE);
+    mx = v > mx ? v : mx;
+  }
+  if (lane == TFF_WAVE - 1) {
+    s_tot[wv] = inc;
+    s_max[wv] = mx;
   }
   __syncthreads();
-  // pair k -> keys (lo, lo + stride); for stride <= 64 the pairs of one wave (k in a run of 64,
-  // blockDim a multiple of 64) touch only that wave's own 128 keys, so a wave barrier suffices
+  int run = inc - sum, big = 0;
+  for (int v = 0; v < SORT_THREADS / TFF_WAVE; ++v) {
+    if (v < wv) run += s_tot[v];
+    big = s_max[v] > big ? s_max[v] : big;
+  }
+  for (int b = b0; b < b1; ++b) {
+    const int c = s_c[b];
+    s_c[b] = run;
+    run += c;
+  }
+  __syncthreads();
+  // place the keys by bucket; afterwards s_c[b] is the end of bucket b
+  for (int k = t; k < m; k += SORT_THREADS) {
+    const unsigned long long key = column_key(coords, sorted_orig[cs + k], g);
+    s_k[atomicAdd(&s_c[column_bucket(key, scale, nb)], 1)] = key;
+  }
+  __syncthreads();
+  if (big <= BUCKET_RANK_MAX) {
+    for (int p = t; p < m; p += SORT_THREADS) {
+      const unsigned long long key = s_k[p];
+      const int b = column_bucket(key, scale, nb);
+      const int s = b > 0 ? s_c[b - 1] : 0, e = s_c[b];
+      int r = 0;
+      for (int q = s; q < e; ++q) r += s_k[q] < key ? 1 : 0;
+      sorted_orig[cs + s + r] = (int)(unsigned)(key & 0xffffffffull);
+    }
+    return;
+  }
+  // bitonic network over the placed keys, padded to a power of two (pair k -> keys (lo, lo + stride))
+  int np = 1;
+  while (np < m) np <<= 1;
+  for (int k = m + t; k < np; k += SORT_THREADS) s_k[k] = ~0ull;
+  __syncthreads();
   for (int size = 2; size <= np; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int k = threadIdx.x; k < (np >> 1); k += blockDim.x) {
+      for (int k = t; k < (np >> 1); k += SORT_THREADS) {
         const int lo = (k & ~(stride - 1)) * 2 + (k & (stride - 1)), hi = lo + stride;
         const bool asc = (lo & size) == 0;
         const unsigned long long x = s_k[lo], y = s_k[hi];
@@ -207,18 +280,10 @@ __global__ __launch_bounds__(256) void k_sort_columns(const TI* __restrict__ coo
           s_k[hi] = x;
         }
       }
-      const int next = stride > 1 ? stride >> 1 : size;
-      if (stride > TFF_WAVE || next > TFF_WAVE) {
-        __syncthreads();
-      } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
+      __syncthreads();
     }
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < m; k += blockDim.x) sorted_orig[cs + k] = (int)(unsigned)(s_k[k] & 0xffffffffull);
+  for (int k = t; k < m; k += SORT_THREADS) sorted_orig[cs + k] = (int)(unsigned)(s_k[k] & 0xffffffffull);
 }
 
 // zero several int32 buffers in one launch (lengths in 32-bit words)
@@ -1833,7 +1898,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
                      w.ncl, status);
   hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
                      w.sorted_orig);
-  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(256), 0, st, coords, g, w.col_start, w.sorted_orig);
+  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(SORT_THREADS), 0, st, coords, g, w.col_start, w.sorted_orig);
   // 2. cluster arrays
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
   hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbW), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
@@ -1942,7 +2007,7 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
                      w.ncl, status);
   hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
                      w.sorted_orig);
-  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(256), 0, st, coords, g, w.col_start, w.sorted_orig);
+  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(SORT_THREADS), 0, st, coords, g, w.col_start, w.sorted_orig);
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
   hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbW), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
                      w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
