@@ -566,8 +566,10 @@ def main():
         sb = pme_stage_bytes(w)
         out["pme_stages"] = {
             "note": "reciprocal space runs on a second stream concurrently with the pair search and pair "
-                    "kernel: stage times are wall times under that sharing (bandwidths are lower bounds); "
-                    "'alone': the same stages run on the caller's stream, one after another",
+                    "kernel (from 300 000 atoms on, the spread runs first on the caller's stream and only "
+                    "the FFT and the gather run beside the pair kernel): stage times are wall times under "
+                    "that sharing (bandwidths are lower bounds); 'alone': the same stages run on the "
+                    "caller's stream, one after another",
             **{st: {"bytes": b, "ms": kern[f"fused_{st}"]["avg_ms"],
                     "GB/s": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9,
                     "frac_of_hbm_peak": b / (kern[f"fused_{st}"]["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}

@@ -43,11 +43,13 @@ int tff_abi_version(void);
  * "fused_spread", "fused_fft", "fused_gather". */
 void tff_prof_enable(int on);
 int tff_prof_collect(const char* name, double* total_ms, long long* count);
-/* Timing aid (no counterpart in the reference): 0 runs the reciprocal-space chain of
- * tff_fused_energy_forces on the caller's stream after the cluster build instead of on the
- * auxiliary stream concurrently with the pair search and pair kernel, so that its stages can be
- * timed alone.  Results are identical either way.  Default 1. */
-void tff_fused_set_overlap(int on);
+/* Schedule of the reciprocal-space chain of tff_fused_energy_forces (no counterpart in the
+ * reference; results are identical in every mode).  0: on the caller's stream after the cluster
+ * build, in order (a timing aid: its stages can be timed alone); 1 (default): by size, as 3 below
+ * 300 000 atoms and as 2 from there on; 2: the charge spread on the caller's stream before the
+ * pair search, then the FFT and the gather on an auxiliary stream concurrently with the pair
+ * search and pair kernel; 3: the whole chain on the auxiliary stream.  Other values mean 1. */
+void tff_fused_set_overlap(int mode);
 
 /* --------------------------------------------------------- neighbor list */
 #define TFF_NBLIST_NSQUARED 0
