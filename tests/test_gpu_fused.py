@@ -72,6 +72,26 @@ def test_fused_deterministic():
     assert np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_fused_schedules_identical(precision):
+    # reciprocal chain serial (0), by size (1), spread before the fork (2), all on the
+    # auxiliary stream (3): integer accumulation makes every schedule bitwise identical
+    from torchff import _lib as L
+    w = make_workload("water900")
+    lib = L.load()
+    outs = {}
+    try:
+        for mode in (1, 0, 2, 3):
+            lib.tff_fused_set_overlap(mode)
+            outs[mode] = _run(w, precision)
+    finally:
+        lib.tff_fused_set_overlap(1)
+    for mode in (0, 2, 3):
+        assert outs[mode][0] == outs[1][0]
+        assert np.array_equal(outs[mode][1], outs[1][1])
+        assert np.array_equal(outs[mode][2], outs[1][2])
+
+
 def test_fused_small_odd_box():
     # 200 waters at 1 g/cm3 (L ~ 1.81 nm would violate 2 rc; use rc 0.8) with an odd grid
     from torchff.workloads import box_length_for_density
