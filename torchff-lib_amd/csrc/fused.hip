@@ -299,7 +299,7 @@ __global__ void k_clear(ClearList c) {
 }
 
 // ---------------------------------------------------------------- 2. cluster arrays
-// One wave per cluster (lanes 0..31 = slots): the cluster takes 32 consecutive atoms of its
+// Two clusters per wave, one per half (lanes 0..31, 32..63 = slots): a cluster takes 32 consecutive atoms of its
 // column's z-sorted range (the last cluster of a column is padded).  The slots are then
 // ordered into four spatially compact groups of 8: sort by x and split in halves, sort each
 // half by y and split, order each group by z (padding slots, keyed +inf, collect at the end;
@@ -318,10 +318,10 @@ __global__ __launch_bounds__(FB) void k_build_clusters(const TI* __restrict__ co
                                                        int* __restrict__ slot_of_orig, int* __restrict__ cl_col,
                                                        float4* __restrict__ bb_c, float4* __restrict__ bb_h,
                                                        float4* __restrict__ sb_c, float4* __restrict__ sb_h) {
-  __shared__ float4 s_p[FB / TFF_WAVE][CL];
-  const int c = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
-  const int lane = threadIdx.x & (TFF_WAVE - 1);
-  const int w = threadIdx.x / TFF_WAVE;
+  __shared__ float4 s_p[FB / CL][CL];
+  const int c = (blockIdx.x * FB + threadIdx.x) / CL;
+  const int s = threadIdx.x & (CL - 1);
+  const int w = threadIdx.x / CL;
   if (c >= *ncl_ptr) return;
   // column of cluster c
   int lo = 0, hi = g.ncol[0] * g.ncol[1];
@@ -330,8 +330,7 @@ __global__ __launch_bounds__(FB) void k_build_clusters(const TI* __restrict__ co
     if (col_cl_start[mid] <= c) lo = mid; else hi = mid;
   }
   const int col = lo;
-  if (lane == 0) cl_col[c] = col;
-  const int s = lane & (CL - 1);
+  if (s == 0) cl_col[c] = col;
   const int pos = col_start[col] + (c - col_cl_start[col]) * CL + s;
   const int o = pos < col_start[col + 1] ? sorted_orig[pos] : -1;
   V4<TC> p;
@@ -350,28 +349,28 @@ __global__ __launch_bounds__(FB) void k_build_clusters(const TI* __restrict__ co
   // rank by x over all 32 slots
   int rx = 0;
   for (int t = 0; t < CL; ++t) {
-    float vx = __shfl(kx, t, TFF_WAVE);
+    float vx = __shfl(kx, t, CL);
     rx += (vx < kx || (vx == kx && t < s)) ? 1 : 0;
   }
   const int half = rx >= CL / 2;
   int ry = 0;
   for (int t = 0; t < CL; ++t) {
-    float vy = __shfl(ky, t, TFF_WAVE);
-    int ht = __shfl(half, t, TFF_WAVE);
+    float vy = __shfl(ky, t, CL);
+    int ht = __shfl(half, t, CL);
     ry += (ht == half && (vy < ky || (vy == ky && t < s))) ? 1 : 0;
   }
   const int grp = 2 * half + (ry >= CL / 4);
   int rz = 0;
   for (int t = 0; t < CL; ++t) {
-    float vz = __shfl(kz, t, TFF_WAVE);
-    int gt = 2 * __shfl(half, t, TFF_WAVE);
-    int rt = __shfl(ry, t, TFF_WAVE);
+    float vz = __shfl(kz, t, CL);
+    int gt = 2 * __shfl(half, t, CL);
+    int rt = __shfl(ry, t, CL);
     gt += rt >= CL / 4;
     rz += (gt == grp && (vz < kz || (vz == kz && t < s))) ? 1 : 0;
   }
   const int nl = grp * 8 + rz;
   const int ns = c * CL + nl;
-  if (lane < CL) {
+  {
     // padding slots sit far away, each at its own place (see PAD_STEP)
     xyzq[ns] = o >= 0 ? p : V4<TC>{(TC)(PAD_STEP * (1.0 + ns)), (TC)PAD_STEP, (TC)PAD_STEP, (TC)0};
     lj[ns] = l;
@@ -382,8 +381,9 @@ __global__ __launch_bounds__(FB) void k_build_clusters(const TI* __restrict__ co
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // bounding boxes of the ordered slots: lane l holds slot l & 31
-  const float4 qv = s_p[w][lane & (CL - 1)];
+  // bounding boxes of the ordered slots: thread s of the half wave holds slot s (the xor
+  // shuffles below stay inside the half)
+  const float4 qv = s_p[w][s];
   const bool v = qv.w != 0.f;
   float mn[3], mx[3], gmn[3], gmx[3];
   const float pv[3] = {qv.x, qv.y, qv.z};
@@ -401,13 +401,13 @@ __global__ __launch_bounds__(FB) void k_build_clusters(const TI* __restrict__ co
       mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, TFF_WAVE));
     }
   }
-  if (lane == 0) {
+  if (s == 0) {
     bb_c[c] = make_float4(0.5f * (mn[0] + mx[0]), 0.5f * (mn[1] + mx[1]), 0.5f * (mn[2] + mx[2]), 0.f);
     bb_h[c] = make_float4(0.5f * (mx[0] - mn[0]) * 1.0001f + 1e-6f, 0.5f * (mx[1] - mn[1]) * 1.0001f + 1e-6f,
                           0.5f * (mx[2] - mn[2]) * 1.0001f + 1e-6f, 0.f);
   }
-  if (lane < CL && (lane & 7) == 0) {
-    const int gi = c * 4 + lane / 8;
+  if ((s & 7) == 0) {
+    const int gi = c * 4 + s / 8;
     if (gmx[0] < gmn[0]) {
       sb_c[gi] = make_float4(0.f, 0.f, 0.f, 0.f);
       sb_h[gi] = make_float4(-1.f, -1.f, -1.f, 0.f);
@@ -1907,7 +1907,8 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(SORT_THREADS), 0, st, coords, g, w.col_start, w.sorted_orig);
   // 2. cluster arrays
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
-  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbW), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
+  const unsigned nbB = (unsigned)(((long long)g.ncl_max * CL + FB - 1) / FB);  // two clusters per wave
+  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbB), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
                      w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
                      w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
   hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
@@ -2018,7 +2019,8 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
                      w.sorted_orig);
   hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(SORT_THREADS), 0, st, coords, g, w.col_start, w.sorted_orig);
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
-  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbW), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
+  const unsigned nbB = (unsigned)(((long long)g.ncl_max * CL + FB - 1) / FB);  // two clusters per wave
+  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbB), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
                      w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
                      w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
   hipLaunchKernelGGL((k_fill_vslots<TI, TC>), dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, gF, (V4<TC>*)w.vsl);
