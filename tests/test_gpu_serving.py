@@ -29,8 +29,9 @@ def _rms_rel(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
 
 
-# 20 waters: edge 0.84 nm < rc; 35: rc < edge < 2 rc; 100: edge 1.44 < 2 rc; 300: edge 2.08 >= 2 rc
-SIZES = [(20, 101), (35, 102), (100, 103), (300, 104)]
+# 20 waters: edge 0.84 nm < rc; 35: rc < edge < 2 rc; 100: edge 1.44 < 2 rc; 300: edge 2.08 >= 2 rc;
+# 1000: the largest request size of the serving stream (edge 3.10 nm, 32^3 grid)
+SIZES = [(20, 101), (35, 102), (100, 103), (300, 104), (1000, 105)]
 
 
 @pytest.fixture(scope="module")
