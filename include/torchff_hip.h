@@ -236,8 +236,14 @@ int tff_cluster_pairs_nonbonded(int dtype, const void* coords, long long natoms,
 /* ------------------------------------------------- fused energy + forces */
 #define TFF_FUSED_PAIR_OVERFLOW 1  /* cluster-pair capacity exceeded: results incomplete */
 #define TFF_FUSED_EXCL_OVERFLOW 2  /* an atom has more than 8 exclusion partners */
-#define TFF_FUSED_COLUMN_OVERFLOW 4 /* an x-y column holds more than 8192 atoms (left unsorted:
-                                       results valid but not bitwise reproducible) */
+#define TFF_FUSED_COLUMN_OVERFLOW 4 /* an x-y column holds more than 8192 atoms: left unsorted in
+                                       z, which the charge spreading's cluster search relies on
+                                       (reciprocal-space results incomplete) */
+#define TFF_FUSED_BRICK_OVERFLOW 8  /* a charge-spreading brick is reached by more than 1024
+                                       clusters (strongly inhomogeneous system or a coarse grid):
+                                       reciprocal-space results incomplete */
+#define TFF_FUSED_BOX_STALE 16      /* box_device no longer holds the values of box_host that the
+                                       call was planned from: results invalid */
 
 size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const double* box_host, double list_cutoff,
                                  int K0, int K1, int K2);
@@ -253,15 +259,19 @@ size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const doub
  * correction), reciprocal, self, total.  forces_out (io_dtype, N x 3) = -dE/dr.  status_out
  * (device, 2 int32): [0] TFF_FUSED_* flags, 0 when complete; [1] number of 32 x 32
  * cluster-pair tiles within the list cutoff (their 8 x 8 sub-tiles are pruned at the
- * cutoff on every call).  Moduli as for tff_pme_energy with zmoduli of length K2/2 + 1, in
- * compute_dtype. */
+ * cutoff on every call).  box_device (optional, io_dtype, 9 values): the caller's device copy
+ * of the box; the call flags TFF_FUSED_BOX_STALE when it differs from box_host.  status_host
+ * (optional, 2 int32 in pinned host memory mapped for the device): the last kernel of the call
+ * copies status_out there, so the caller can read the flags once the call's work has completed
+ * (an event) without a device-to-host copy.  Moduli as for tff_pme_energy with zmoduli of length
+ * K2/2 + 1, in compute_dtype. */
 int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
                             const void* sigma, const void* epsilon, long long natoms, const double* box_host,
                             const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff, double list_cutoff,
                             double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
                             const void* xmoduli, const void* ymoduli, const void* zmoduli, double* energies_out,
-                            void* forces_out, int32_t* status_out, void* workspace, size_t workspace_bytes,
-                            tff_stream_t stream);
+                            void* forces_out, int32_t* status_out, const void* box_device, int32_t* status_host,
+                            void* workspace, size_t workspace_bytes, tff_stream_t stream);
 
 /* Vector-Jacobian product of tff_fused_energy_forces, for force matching: given upstream
  * gradients grad_energy (device, 1 double: dL/dE; NULL = 0) and grad_forces (device, io_dtype
@@ -284,8 +294,8 @@ int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void
                                      const void* xmoduli, const void* ymoduli, const void* zmoduli,
                                      const double* grad_energy, const void* grad_forces, void* grad_coords,
                                      void* grad_charges, void* grad_sigma, void* grad_epsilon,
-                                     int32_t* status_out, void* workspace, size_t workspace_bytes,
-                                     tff_stream_t stream);
+                                     int32_t* status_out, const void* box_device, int32_t* status_host,
+                                     void* workspace, size_t workspace_bytes, tff_stream_t stream);
 
 /* ----------------------------------------------------- batched serving */
 /* Energy and forces of many small orthorhombic water-like systems in one fixed sequence of

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_workspace_queries_and_errors_without_gpu():
     lib = _lib.load(require_gpu=False)
-    assert lib.tff_abi_version() == 1
+    assert lib.tff_abi_version() == 2
     box = (ctypes.c_double * 9)(9.86, 0, 0, 0, 9.86, 0, 0, 0, 9.86)
     assert lib.tff_nblist_workspace_bytes(96000, box, 1.1, 1) > 96000 * 64
     assert lib.tff_pair_workspace_bytes(1, 96000, 10, 0) >= 96000 * 8 * 5
@@ -107,3 +107,17 @@ def test_workloads_are_reproducible():
     ang = np.degrees(np.arccos(np.sum((c[:, 1] - c[:, 0]) * (c[:, 2] - c[:, 0]), 1) / d1 ** 2))
     np.testing.assert_allclose(d1, 0.09572, atol=1e-12)
     np.testing.assert_allclose(ang, 104.52, atol=1e-9)
+
+
+def test_host_box_is_read_on_every_call():
+    # a host box is never cached: a write through .data (which autograd does not version) is seen
+    b = torch.eye(3, dtype=torch.float64) * 3.0
+    assert _lib.box_numpy(b)[2, 2] == 3.0
+    b.data.mul_(1.01)
+    assert _lib.box_numpy(b)[2, 2] == 3.0 * 1.01
+    b[0, 0] = 5.0
+    assert _lib.box_numpy(b)[0, 0] == 5.0
+    # the cache switch and invalidation are plain host state
+    torchff.set_box_cache(False)
+    torchff.set_box_cache(True)
+    torchff.invalidate_box_cache(b)

@@ -61,7 +61,7 @@ void tff_prof_end(void* token, hipStream_t stream) {
 
 extern "C" const char* tff_last_error(void) { return g_err; }
 
-extern "C" int tff_abi_version(void) { return 1; }
+extern "C" int tff_abi_version(void) { return 2; }
 
 extern "C" void tff_prof_enable(int on) {
   std::lock_guard<std::mutex> lk(g_prof_mu);

@@ -85,10 +85,18 @@ __device__ __forceinline__ void wrap_coords(const TI* __restrict__ coords, long 
 template <typename TI>
 __global__ __launch_bounds__(FB) void k_bin_atoms(const TI* __restrict__ coords, const TI* __restrict__ q, long long n,
                                                   Geo g, int* __restrict__ col_of, int* __restrict__ off_in_col,
-                                                  int* __restrict__ col_count, double* __restrict__ q2part) {
+                                                  int* __restrict__ col_count, double* __restrict__ q2part,
+                                                  const TI* __restrict__ box_dev, int* __restrict__ status) {
   const long long i = blockIdx.x * (long long)FB + threadIdx.x;
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const bool valid = i < n;
+  if (box_dev && blockIdx.x == 0 && threadIdx.x == 0) {
+    // the call was planned (cell grid, capacities, PME geometry) from a host copy of the box: a
+    // device box that no longer holds those values marks every result of the call as invalid
+    bool same = true;
+    for (int k = 0; k < 9; ++k) same = same && box_dev[k] == (k % 4 == 0 ? (TI)g.L[k / 4] : (TI)0);
+    if (!same) atomicOr(status, TFF_FUSED_BOX_STALE);
+  }
   {
     // sum of q^2 of the block (self energy), one partial per block
     const double qi = valid ? (double)q[i] : 0.0;
@@ -1305,7 +1313,7 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
   }
   int ncand = s_n;
   if (ncand > BRICK_MAXC) {
-    if (threadIdx.x == 0) atomicOr(status, TFF_FUSED_PAIR_OVERFLOW);
+    if (threadIdx.x == 0) atomicOr(status, TFF_FUSED_BRICK_OVERFLOW);
     ncand = BRICK_MAXC;
   }
   // Candidate clusters in chunks of SPREAD_CHUNK: the slots whose stencil reaches the brick are
@@ -1570,8 +1578,14 @@ __global__ __launch_bounds__(FB) void k_finalize_grad(const int* __restrict__ nc
                                                       const unsigned long long* __restrict__ gpme,
                                                       const TI* __restrict__ q, double coul, double alpha,
                                                       const double* __restrict__ gE_ptr, TI* __restrict__ dr,
-                                                      TI* __restrict__ dq, TI* __restrict__ dsig, TI* __restrict__ deps) {
+                                                      TI* __restrict__ dq, TI* __restrict__ dsig, TI* __restrict__ deps,
+                                                      const int* __restrict__ status, int* __restrict__ status_host) {
   const int sl = blockIdx.x * FB + threadIdx.x;
+  if (status_host && sl == 0) {
+    // every kernel that sets a flag ran before this one on the stream
+    status_host[0] = status[0];
+    status_host[1] = status[1];
+  }
   if (sl >= *ncl_ptr * CL) return;
   const int i = slot_orig[sl];
   if (i < 0) return;
@@ -1598,8 +1612,14 @@ __global__ __launch_bounds__(FB) void k_finalize(const int* __restrict__ ncl_ptr
                                                  const unsigned long long* __restrict__ eacc,
                                                  const double* __restrict__ kparts, int nk,
                                                  const double* __restrict__ qparts, int nq, double coul,
-                                                 double alpha, double* __restrict__ out) {
+                                                 double alpha, double* __restrict__ out, const int* __restrict__ status,
+                                                 int* __restrict__ status_host) {
   const int sl = blockIdx.x * FB + threadIdx.x;
+  if (status_host && sl == 0) {
+    // every kernel that sets a flag ran before this one on the stream (after the join)
+    status_host[0] = status[0];
+    status_host[1] = status[1];
+  }
   if (sl < *ncl_ptr * CL) {
     const int i = slot_orig[sl];
     if (i >= 0)
@@ -1873,7 +1893,8 @@ template <typename TI, typename TC>
 int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long long n, const TffBox& box,
               const Geo& g, const int64_t* row_ptr, const int64_t* col, double cutoff, double list_cutoff,
               double alpha, double coul, const int K[3], int order, const TC* mx, const TC* my, const TC* mz,
-              double* energies, TI* forces, int* status, const FusedWs& w, hipStream_t st) {
+              double* energies, TI* forces, int* status, const TI* box_dev, int* status_host, const FusedWs& w,
+              hipStream_t st) {
   if (order < 3 || order > 8) {  // checked before any launch: the stream fork below must be joined
     tff_set_error("fused: unsupported spline order %d (3..8)", order);
     return TFF_ERR_UNSUPPORTED;
@@ -1899,7 +1920,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   }
   // 1. sort
   hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, q, n, g, w.col_of, w.off_in_col,
-                     w.col_count, w.qparts);
+                     w.col_count, w.qparts, box_dev, status);
   hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.col_count, ncols, w.col_start, w.col_cl_start,
                      w.ncl, status);
   hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
@@ -1976,7 +1997,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
   // 6. outputs, after the reciprocal-space work
   aux_stream_join(st, sp);
   hipLaunchKernelGGL(k_finalize<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.facc, w.facc_pme, forces, w.eacc,
-                     w.kparts, w.nk, w.qparts, w.nq, coul, alpha, energies);
+                     w.kparts, w.nk, w.qparts, w.nq, coul, alpha, energies, status, status_host);
   TFF_CHECK_LAUNCH("fused energy+forces");
   return TFF_OK;
 }
@@ -1987,8 +2008,8 @@ template <typename TI, typename TC>
 int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, long long n, const TffBox& box,
                    const Geo& g, const int64_t* row_ptr, const int64_t* col, double cutoff, double list_cutoff,
                    double alpha, double coul, const int K[3], int order, const TC* mx, const TC* my, const TC* mz,
-                   const double* gE, const TI* gF, TI* dr, TI* dq, TI* dsig, TI* deps, int* status, const FusedWs& w,
-                   hipStream_t st) {
+                   const double* gE, const TI* gF, TI* dr, TI* dq, TI* dsig, TI* deps, int* status, const TI* box_dev,
+                   int* status_host, const FusedWs& w, hipStream_t st) {
   if (order < 3 || order > 8) {
     tff_set_error("fused: unsupported spline order %d (3..8)", order);
     return TFF_ERR_UNSUPPORTED;
@@ -2012,7 +2033,7 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, cl);
   }
   hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, q, n, g, w.col_of, w.off_in_col,
-                     w.col_count, w.qparts);
+                     w.col_count, w.qparts, box_dev, status);
   hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.col_count, ncols, w.col_start, w.col_cl_start,
                      w.ncl, status);
   hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
@@ -2079,7 +2100,7 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
   }
 #undef TFF_GRAD_PME
   hipLaunchKernelGGL(k_finalize_grad<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.gacc, w.gpme, q, coul,
-                     alpha, gE, dr, dq, dsig, deps);
+                     alpha, gE, dr, dq, dsig, deps, status, status_host);
   TFF_CHECK_LAUNCH("fused energy+forces backward");
   return TFF_OK;
 }
@@ -2145,7 +2166,8 @@ extern "C" int tff_fused_energy_forces_backward(
     double list_cutoff, double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
     const void* xmoduli, const void* ymoduli, const void* zmoduli, const double* grad_energy,
     const void* grad_forces, void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon,
-    int32_t* status_out, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    int32_t* status_out, const void* box_device, int32_t* status_host, void* workspace, size_t workspace_bytes,
+    hipStream_t stream) {
   TffBox box = tff_make_box(box_host);
   Geo g;
   int K[3] = {K0, K1, K2};
@@ -2167,7 +2189,8 @@ extern "C" int tff_fused_energy_forces_backward(
                                 natoms, box, g, excl_row_ptr, excl_col, cutoff, list_cutoff, ewald_alpha,          \
                                 coul_constant, K, order, (const TC*)xmoduli, (const TC*)ymoduli,                  \
                                 (const TC*)zmoduli, grad_energy, (const TI*)grad_forces, (TI*)grad_coords,         \
-                                (TI*)grad_charges, (TI*)grad_sigma, (TI*)grad_epsilon, (int*)status_out, w, stream)
+                                (TI*)grad_charges, (TI*)grad_sigma, (TI*)grad_epsilon, (int*)status_out,        \
+                                (const TI*)box_device, (int*)status_host, w, stream)
   if (io_dtype == TFF_F64 && compute_dtype == TFF_F32) TFF_RUN(double, float);
   if (io_dtype == TFF_F32 && compute_dtype == TFF_F32) TFF_RUN(float, float);
   if (io_dtype == TFF_F64 && compute_dtype == TFF_F64) TFF_RUN(double, double);
@@ -2184,7 +2207,8 @@ extern "C" int tff_fused_energy_forces(int io_dtype, int compute_dtype, const vo
                                        double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
                                        const void* xmoduli, const void* ymoduli, const void* zmoduli,
                                        double* energies_out, void* forces_out, int32_t* status_out,
-                                       void* workspace, size_t workspace_bytes, hipStream_t stream) {
+                                       const void* box_device, int32_t* status_host, void* workspace,
+                                       size_t workspace_bytes, hipStream_t stream) {
   TffBox box = tff_make_box(box_host);
   Geo g;
   if (!make_geo(box, natoms, list_cutoff, g)) {
@@ -2225,7 +2249,7 @@ extern "C" int tff_fused_energy_forces(int io_dtype, int compute_dtype, const vo
   return run_fused<TI, TC>((const TI*)coords, (const TI*)charges, (const TI*)sigma, (const TI*)epsilon, natoms, \
                            box, g, excl_row_ptr, excl_col, cutoff, list_cutoff, ewald_alpha, coul_constant, K,   \
                            order, (const TC*)xmoduli, (const TC*)ymoduli, (const TC*)zmoduli, energies_out,      \
-                           (TI*)forces_out, (int*)status_out, w, stream)
+                           (TI*)forces_out, (int*)status_out, (const TI*)box_device, (int*)status_host, w, stream)
   if (io_dtype == TFF_F64 && compute_dtype == TFF_F32) TFF_RUN(double, float);
   if (io_dtype == TFF_F32 && compute_dtype == TFF_F32) TFF_RUN(float, float);
   if (io_dtype == TFF_F64 && compute_dtype == TFF_F64) TFF_RUN(double, double);

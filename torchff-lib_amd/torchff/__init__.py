@@ -8,6 +8,7 @@ terms and smooth PME: the same module classes, functional wrappers and
 """
 
 from . import _ops  # noqa: F401  (registers torch.ops.torchff.*)
+from ._lib import set_box_cache, invalidate_box_cache  # noqa: F401
 from .pbc import PBC, applyPBC  # noqa: F401
 from .nblist import (  # noqa: F401
     NeighborList, build_neighbor_list_nsquared, build_neighbor_list_cell_list, build_cluster_pairs,

@@ -58,11 +58,11 @@ _SIGNATURES = {
     "tff_ewald": (_i, [_i, _vp, _vp, _ll, _vp, _i, _d, _vp, _vp, _vp, _vp, _sz, _vp]),
     "tff_fused_workspace_bytes": (_sz, [_i, _ll, _vp, _d, _i, _i, _i]),
     "tff_fused_energy_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i, _i, _i,
-                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "tff_fused_backward_workspace_bytes": (_sz, [_i, _ll, _vp, _d, _i, _i, _i]),
     "tff_fused_energy_forces_backward": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i,
-                                              _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz,
-                                              _vp]),
+                                              _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                              _vp, _sz, _vp]),
     "tff_serve_table_bytes": (_sz, [_i, _i]),
     "tff_serve_atoms_offset": (_sz, [_i]),
     "tff_serve_table_reset": (None, [_vp]),
@@ -129,25 +129,52 @@ def require_cuda(*tensors):
                                f"got one on {t.device}")
 
 
-def box_host(box):
-    """float64 host copy (9,) of a (3,3) box, cached on the tensor object per in-place version.
+_box_cache = True
 
-    A device box is copied to the host once per new box tensor (or after it is
-    modified in place); reusing the same box tensor costs nothing afterwards,
-    which keeps repeated calls free of device-to-host synchronisation.
+
+def set_box_cache(enabled: bool):
+    """Switch the host copy cache of device boxes on (default) or off.  Off: every call copies the
+    box to the host (one device-to-host synchronisation per call, not graph-capturable)."""
+    global _box_cache
+    _box_cache = bool(enabled)
+
+
+def invalidate_box_cache(box):
+    """Forget the cached host copy of ``box`` (after writing it in a way autograd does not
+    track: through ``.data``, a DLPack or numpy view, or a replayed graph)."""
+    try:
+        del box._tff_host_box
+    except AttributeError:
+        pass
+
+
+def box_host(box):
+    """float64 host copy (9,) of a (3,3) box.
+
+    A host (CPU) box is read on every call.  A device box is copied to the host
+    once and cached on the tensor object, keyed by its data pointer and its
+    in-place version counter, so repeated calls with the same box stay free of
+    device-to-host synchronisation (and can be captured into a graph).  Writes
+    that bypass the version counter (``.data``, DLPack / numpy views, graph
+    replays) are not seen by the cache: call :func:`invalidate_box_cache`, or
+    :func:`set_box_cache` (False).  The fused path also compares the device box
+    with the cached copy on the device and reports a mismatch (TFF_FUSED_BOX_STALE).
     """
     if box is None:
         return None
-    ver = box._version
-    cache = getattr(box, "_tff_host_box", None)
-    if cache is not None and cache[0] == ver:
-        return cache[1]
+    cached = _box_cache and box.is_cuda
+    if cached:
+        key = (box.data_ptr(), box._version)
+        cache = getattr(box, "_tff_host_box", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
     b = box.detach().to("cpu", torch.float64).contiguous().reshape(9).numpy().copy()
     arr = (ctypes.c_double * 9)(*b.tolist())
-    try:
-        box._tff_host_box = (ver, arr)
-    except (AttributeError, RuntimeError):
-        pass
+    if cached:
+        try:
+            box._tff_host_box = (key, arr)
+        except (AttributeError, RuntimeError):
+            pass
     return arr
 
 

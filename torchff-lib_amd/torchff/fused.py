@@ -21,6 +21,7 @@ cluster pair list; it is not itself differentiable.
 """
 
 import math
+from collections import deque
 
 import torch
 import torch.nn as nn
@@ -33,6 +34,26 @@ from .workloads import COULOMB_KJMOL
 PAIR_OVERFLOW = 1
 EXCL_OVERFLOW = 2
 COLUMN_OVERFLOW = 4
+BRICK_OVERFLOW = 8
+BOX_STALE = 16
+# every flag means the call's energies and forces are wrong
+INVALID = PAIR_OVERFLOW | EXCL_OVERFLOW | COLUMN_OVERFLOW | BRICK_OVERFLOW | BOX_STALE
+_MESSAGES = (
+    (PAIR_OVERFLOW, "cluster-pair capacity exceeded (neighbour count per cluster far above the box average)"),
+    (EXCL_OVERFLOW, "an atom has more than 8 exclusion partners"),
+    (BRICK_OVERFLOW, "a charge-spreading brick is reached by more than 1024 clusters (strongly inhomogeneous "
+                     "system or a grid much coarser than the atom spacing)"),
+    (BOX_STALE, "the box tensor was changed without a version bump (through .data, a view or a graph "
+                "replay) after its host copy was cached; call torchff.invalidate_box_cache(box) or "
+                "torchff.set_box_cache(False)"),
+    (COLUMN_OVERFLOW, "an x-y column holds more than 8192 atoms (strongly inhomogeneous system): it is left "
+                      "unsorted in z, which the charge spreading's cluster search relies on"),
+)
+_RING = 8  # calls whose status may be outstanding at once
+
+
+def status_message(flags):
+    return "; ".join(m for bit, m in _MESSAGES if flags & bit)
 
 
 class FusedNonbondedPME(nn.Module):
@@ -66,6 +87,12 @@ class FusedNonbondedPME(nn.Module):
         self._ws_key = None
         self._mod = None
         self.status = None
+        # status words of recent calls, mirrored into pinned host memory by each call's last kernel:
+        # (event, host tensor, what) in call order; read once the event has completed, so that an
+        # overflow surfaces on a later call (or at check()) without synchronising the call itself
+        self._ring = None
+        self._ring_next = 0
+        self._pending = deque()
 
     def _workspace(self, box_host, device):
         lib = L.load()
@@ -79,10 +106,51 @@ class FusedNonbondedPME(nn.Module):
             self._ws_key = key
         return self._ws
 
+    def _status_slot(self, device):
+        """(device status tensor, pinned host mirror) for a new call, after reading the status of
+        every earlier call that has completed (raising if one of them was invalid)."""
+        self._poll(block=len(self._pending) >= _RING)
+        if self._ring is None:
+            self._ring = torch.zeros((_RING, 2), dtype=torch.int32, pin_memory=True)
+        host = self._ring[self._ring_next]
+        self._ring_next = (self._ring_next + 1) % _RING
+        return torch.empty(2, dtype=torch.int32, device=device), host
+
+    def _record(self, host, what, stream_tensor):
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(stream_tensor.device))
+        self._pending.append((ev, host, what))
+
+    def _poll(self, block=False, all_=False):
+        """Read the status of completed calls in call order; raise on the first invalid one.
+        block: wait for the oldest call; all_: wait for every call."""
+        while self._pending:
+            ev, host, what = self._pending[0]
+            if all_ or (block and len(self._pending) >= _RING):
+                ev.synchronize()
+            elif not ev.query():
+                break
+            self._pending.popleft()
+            flags = int(host[0])
+            if flags & INVALID:
+                self._pending.clear()
+                raise RuntimeError(f"FusedNonbondedPME: an earlier {what} returned invalid results: "
+                                   f"{status_message(flags)}")
+
+    def _box_device(self, box, io):
+        b = box.detach()
+        if b.dtype != io or not b.is_contiguous():
+            b = b.to(io).contiguous()
+        return b
+
     @torch._dynamo.disable
     def forward(self, coords, box, charges, sigma, epsilon, check: bool = False):
         """Returns (energy float64 scalar, forces shaped like coords, terms float64 (4,) =
-        [Lennard-Jones, real-space Coulomb incl. exclusion correction, reciprocal, self])."""
+        [Lennard-Jones, real-space Coulomb incl. exclusion correction, reciprocal, self]).
+
+        Capacity overflows and a stale box cache are reported: at once with check=True (one
+        synchronisation), otherwise by raising on a later call as soon as this call's status has
+        arrived on the host (no synchronisation), and by check()."""
         if torch.is_grad_enabled() and any(t.requires_grad for t in (coords, charges, sigma, epsilon)):
             E, F, terms = _FusedEnergyForces.apply(self, coords, charges, sigma, epsilon, box)
             if check:
@@ -111,14 +179,16 @@ class FusedNonbondedPME(nn.Module):
         rp, col = self.excl_row_ptr, self.excl_col_indices
         out = torch.empty(5, dtype=torch.float64, device=dev)
         forces = torch.empty_like(c)
-        status = torch.empty(2, dtype=torch.int32, device=dev)
+        status, host = self._status_slot(dev)
+        bd = self._box_device(box, io)
         cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
         rc = lib.tff_fused_energy_forces(L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms,
                                          bh, L.ptr(rp), L.ptr(col), self.cutoff, self.list_cutoff, self.alpha,
                                          self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
-                                         L.ptr(out), L.ptr(forces), L.ptr(status), L.ptr(ws), ws.numel(),
-                                         L.stream_of(c))
+                                         L.ptr(out), L.ptr(forces), L.ptr(status), L.ptr(bd), L.ptr(host),
+                                         L.ptr(ws), ws.numel(), L.stream_of(c))
         L.check(rc, "FusedNonbondedPME")
+        self._record(host, "forward call", c)
         self.status = status
         if check:
             self.check()
@@ -143,15 +213,17 @@ class FusedNonbondedPME(nn.Module):
         gf = None if grad_forces is None else grad_forces.detach().to(device=dev, dtype=io).contiguous()
         outs = [torch.empty_like(c) if needs[0] else None] + \
                [torch.empty(self.natoms, dtype=io, device=dev) if n else None for n in needs[1:]]
-        status = torch.empty(2, dtype=torch.int32, device=dev)
+        status, host = self._status_slot(dev)
+        bd = self._box_device(box, io)
         rp, col = self.excl_row_ptr, self.excl_col_indices
         rc = lib.tff_fused_energy_forces_backward(
             L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms, bh, L.ptr(rp), L.ptr(col),
             self.cutoff, self.list_cutoff, self.alpha, self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
-            L.ptr(ge), L.ptr(gf), *(L.ptr(o) for o in outs), L.ptr(status), L.ptr(self._bws), self._bws.numel(),
-            L.stream_of(c))
+            L.ptr(ge), L.ptr(gf), *(L.ptr(o) for o in outs), L.ptr(status), L.ptr(bd), L.ptr(host),
+            L.ptr(self._bws), self._bws.numel(), L.stream_of(c))
         L.check(rc, "FusedNonbondedPME backward")
-        self.status = status
+        self._record(host, "backward call", c)
+        self.backward_status = status
         return outs
 
     def tiles(self):
@@ -159,17 +231,15 @@ class FusedNonbondedPME(nn.Module):
         return None if self.status is None else int(self.status[1].item())
 
     def check(self):
-        """Raise if the last call overflowed an internal capacity (synchronises)."""
-        if self.status is None:
-            return
-        st = int(self.status[0].item())
-        if st & PAIR_OVERFLOW:
-            raise RuntimeError("FusedNonbondedPME: cluster-pair capacity exceeded")
-        if st & EXCL_OVERFLOW:
-            raise RuntimeError("FusedNonbondedPME: an atom has more than 8 exclusion partners")
-        if st & COLUMN_OVERFLOW:
-            raise RuntimeError("FusedNonbondedPME: an x-y column holds more than 8192 atoms (strongly "
-                               "inhomogeneous system); results are not reproducible bit for bit")
+        """Wait for every outstanding call (forward and backward) and raise if one of them
+        overflowed an internal capacity or met a stale box cache."""
+        while self._pending:
+            ev, host, what = self._pending.popleft()
+            ev.synchronize()
+            flags = int(host[0])
+            if flags:
+                self._pending.clear()
+                raise RuntimeError(f"FusedNonbondedPME: {what}: {status_message(flags)}")
 
 
 class _FusedEnergyForces(torch.autograd.Function):
