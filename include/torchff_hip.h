@@ -273,6 +273,25 @@ int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords,
                             void* forces_out, int32_t* status_out, const void* box_device, int32_t* status_host,
                             void* workspace, size_t workspace_bytes, tff_stream_t stream);
 
+/* Allocation-free variant of tff_fused_energy_forces (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture with allocation
+ * forbidden, tight serving or MD loops).  Same arguments and results, without the workspace: the
+ * call uses a static device array of TFF_FUSED_SCRATCH_BYTES per device, held from the
+ * library's first use on that device until the process ends, and calls no allocating routine
+ * (the first call on a device looks up the array's address and creates the auxiliary stream).
+ * A system whose tff_fused_workspace_bytes exceeds that size is refused with TFF_ERR_WORKSPACE
+ * (64 MiB: the 96,000-atom water box takes 35 MiB in float32 and 44 MiB in float64 arithmetic).
+ * All calls on one device share the array: calls that overlap on one device race. */
+#define TFF_FUSED_SCRATCH_BYTES (64ull << 20)
+int tff_fused_energy_forces_noalloc(int io_dtype, int compute_dtype, const void* coords, const void* charges,
+                                    const void* sigma, const void* epsilon, long long natoms,
+                                    const double* box_host, const int64_t* excl_row_ptr, const int64_t* excl_col,
+                                    double cutoff, double list_cutoff, double ewald_alpha, double coul_constant,
+                                    int K0, int K1, int K2, int order, const void* xmoduli, const void* ymoduli,
+                                    const void* zmoduli, double* energies_out, void* forces_out,
+                                    int32_t* status_out, const void* box_device, int32_t* status_host,
+                                    tff_stream_t stream);
+
 /* Vector-Jacobian product of tff_fused_energy_forces, for force matching: given upstream
  * gradients grad_energy (device, 1 double: dL/dE; NULL = 0) and grad_forces (device, io_dtype
  * N x 3: dL/dF; NULL = 0), returns dL/dcoords, dL/dcharges, dL/dsigma, dL/depsilon (device,

@@ -1811,6 +1811,17 @@ std::atomic<int> g_overlap{1};
 
 // *sp: the auxiliary stream, waiting for the work enqueued so far on st (st itself when the
 // overlap is off); false when the stream or its events could not be set up
+// creates the auxiliary stream and its events of device dev once (caller holds g_aux_mu)
+bool aux_stream_setup_locked(int dev) {
+  AuxStream& a = g_aux[dev];
+  if (!a.s) {
+    if (hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking) != hipSuccess) return false;
+    if (hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess) return false;
+    if (hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) return false;
+  }
+  return true;
+}
+
 bool aux_stream_fork(hipStream_t st, hipStream_t* sp) {
   *sp = st;
   if (!g_overlap.load()) return true;
@@ -1818,11 +1829,7 @@ bool aux_stream_fork(hipStream_t st, hipStream_t* sp) {
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
   std::lock_guard<std::mutex> lk(g_aux_mu);
   AuxStream& a = g_aux[dev];
-  if (!a.s) {
-    if (hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking) != hipSuccess) return false;
-    if (hipEventCreateWithFlags(&a.fork, hipEventDisableTiming) != hipSuccess) return false;
-    if (hipEventCreateWithFlags(&a.join, hipEventDisableTiming) != hipSuccess) return false;
-  }
+  if (!aux_stream_setup_locked(dev)) return false;
   if (hipEventRecord(a.fork, st) != hipSuccess) return false;
   if (hipStreamWaitEvent(a.s, a.fork, 0) != hipSuccess) return false;
   *sp = a.s;
@@ -2200,7 +2207,8 @@ extern "C" int tff_fused_energy_forces_backward(
   return TFF_ERR_ARG;
 }
 
-extern "C" int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
+namespace {
+int fused_forward(int io_dtype, int compute_dtype, const void* coords, const void* charges,
                                        const void* sigma, const void* epsilon, long long natoms,
                                        const double* box_host, const int64_t* excl_row_ptr,
                                        const int64_t* excl_col, double cutoff, double list_cutoff,
@@ -2257,4 +2265,86 @@ extern "C" int tff_fused_energy_forces(int io_dtype, int compute_dtype, const vo
 #undef TFF_RUN
   tff_set_error("fused: unsupported dtype combination");
   return TFF_ERR_ARG;
+}
+}  // namespace
+
+extern "C" int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
+                                       const void* sigma, const void* epsilon, long long natoms,
+                                       const double* box_host, const int64_t* excl_row_ptr,
+                                       const int64_t* excl_col, double cutoff, double list_cutoff,
+                                       double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
+                                       const void* xmoduli, const void* ymoduli, const void* zmoduli,
+                                       double* energies_out, void* forces_out, int32_t* status_out,
+                                       const void* box_device, int32_t* status_host, void* workspace,
+                                       size_t workspace_bytes, hipStream_t stream) {
+  return fused_forward(io_dtype, compute_dtype, coords, charges, sigma, epsilon, natoms, box_host, excl_row_ptr,
+                       excl_col, cutoff, list_cutoff, ewald_alpha, coul_constant, K0, K1, K2, order, xmoduli,
+                       ymoduli, zmoduli, energies_out, forces_out, status_out, box_device, status_host, workspace,
+                       workspace_bytes, stream);
+}
+
+// ---------------------------------------------------------------- allocation-free variant
+// The workspace of tff_fused_energy_forces_noalloc is this array: each device holds one copy for
+// the life of the process.  Host code never passes the array's name to a kernel (on the host it
+// is only a stand-in for the symbol): the entry point looks up the device address of its copy
+// with hipGetSymbolAddress once per device, at its first call there, and uses that address.
+// Every call on one device shares the copy (and keeps the previous call's contents; every kernel
+// of the call clears or overwrites what it reads first), so calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[TFF_FUSED_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64];
+
+// device address of this device's scratch copy and the auxiliary stream, set up once per device
+int scratch_setup(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    tff_set_error("fused noalloc: no current device");
+    return TFF_ERR_LAUNCH;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        tff_set_error("fused noalloc: hipGetSymbolAddress of the scratch array failed (%s)", hipGetErrorString(e));
+        return TFF_ERR_LAUNCH;
+      }
+      g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+  }
+  std::lock_guard<std::mutex> lk(g_aux_mu);
+  if (!aux_stream_setup_locked(dev)) {
+    tff_set_error("fused noalloc: could not create the auxiliary stream");
+    return TFF_ERR_LAUNCH;
+  }
+  return TFF_OK;
+}
+}  // namespace
+
+extern "C" int tff_fused_energy_forces_noalloc(int io_dtype, int compute_dtype, const void* coords,
+                                               const void* charges, const void* sigma, const void* epsilon,
+                                               long long natoms, const double* box_host,
+                                               const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff,
+                                               double list_cutoff, double ewald_alpha, double coul_constant, int K0,
+                                               int K1, int K2, int order, const void* xmoduli, const void* ymoduli,
+                                               const void* zmoduli, double* energies_out, void* forces_out,
+                                               int32_t* status_out, const void* box_device, int32_t* status_host,
+                                               hipStream_t stream) {
+  const size_t need = tff_fused_workspace_bytes(compute_dtype, natoms, box_host, list_cutoff, K0, K1, K2);
+  if (need > TFF_FUSED_SCRATCH_BYTES) {
+    tff_set_error("fused noalloc: this system needs %zu bytes of scratch, above the %zu of the static array "
+                  "(use tff_fused_energy_forces with a workspace)", need, (size_t)TFF_FUSED_SCRATCH_BYTES);
+    return TFF_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  const int rc = scratch_setup(&ws);
+  if (rc) return rc;
+  return fused_forward(io_dtype, compute_dtype, coords, charges, sigma, epsilon, natoms, box_host, excl_row_ptr,
+                       excl_col, cutoff, list_cutoff, ewald_alpha, coul_constant, K0, K1, K2, order, xmoduli,
+                       ymoduli, zmoduli, energies_out, forces_out, status_out, box_device, status_host, ws,
+                       (size_t)TFF_FUSED_SCRATCH_BYTES, stream);
 }

@@ -59,6 +59,8 @@ _SIGNATURES = {
     "tff_fused_workspace_bytes": (_sz, [_i, _ll, _vp, _d, _i, _i, _i]),
     "tff_fused_energy_forces": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i, _i, _i,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "tff_fused_energy_forces_noalloc": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i,
+                                             _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tff_fused_backward_workspace_bytes": (_sz, [_i, _ll, _vp, _d, _i, _i, _i]),
     "tff_fused_energy_forces_backward": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i,
                                               _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
