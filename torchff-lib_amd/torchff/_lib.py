@@ -12,6 +12,7 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtorchff_hip.so")
+OPS_PATH = os.path.join(_HERE, "libtorchff_ops.so")
 
 F32, F64 = 0, 1
 NBLIST_NSQUARED, NBLIST_CELL_LIST = 0, 1

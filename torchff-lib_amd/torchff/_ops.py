@@ -8,6 +8,13 @@ csrc/vdw/vdw_interface.cpp:6, csrc/pme/pme_interface.cpp:6-9), so code that call
 hard-wired to order 6), and ``compute_nonbonded_ewald_energy_from_atom_pairs``
 is the fused real-space erfc-Coulomb + Lennard-Jones op with exclusions.
 
+The operators the reference registers in C++ (neighbour lists, atom-pair nonbonded / multipolar /
+Lennard-Jones energies and forces, point-charge PME) are registered in C++ here too, by
+libtorchff_ops.so (csrc/torch_ops.cpp), which this module loads: a traced or scripted model
+then resolves them in a process without Python.  The rest is registered below: the cluster-pair
+list ops, PME for point dipoles and quadrupoles (reached through pme_long_range) and with fields,
+and the direct Ewald sum.
+
 Gradient semantics follow the reference's autograd Functions: the forward
 kernel also computes d(energy)/d(input) for every input that requires grad,
 and backward scales those by the incoming gradient
@@ -15,200 +22,20 @@ and backward scales those by the incoming gradient
 pme.cu:15-248 where PME's backward runs the gather).
 """
 
+import os
+
 import torch
 
 from . import _lib as L
 
-_def = torch.library.Library("torchff", "DEF")
+if not os.path.exists(L.OPS_PATH):
+    raise RuntimeError(f"{L.OPS_PATH} is not built; run `python torchff-lib_amd/build_lib.py` "
+                       "(or __graft_entry__.build())")
+torch.ops.load_library(L.OPS_PATH)
+
+_def = torch.library.Library("torchff", "FRAGMENT")
 _cuda = torch.library.Library("torchff", "IMPL", "CUDA")
 _auto = torch.library.Library("torchff", "IMPL", "CompositeImplicitAutograd")
-
-# ---------------------------------------------------------------- neighbor list
-_def.define("build_neighbor_list_nsquared(Tensor coords, Tensor? box, Scalar cutoff, Scalar max_npairs, "
-            "Tensor? excl_row_ptr, Tensor? excl_col_indices, bool include_self) -> (Tensor, Tensor)")
-_def.define("build_neighbor_list_nsquared_out(Tensor coords, Tensor? box, Scalar cutoff, Tensor(a!) pairs, "
-            "Tensor? excl_row_ptr, Tensor? excl_col_indices, bool include_self) -> Tensor")
-_def.define("build_neighbor_list_cell_list(Tensor coords, Tensor? box, Scalar cutoff, Scalar max_npairs, "
-            "Tensor? excl_row_ptr, Tensor? excl_col_indices, bool include_self) -> (Tensor, Tensor)")
-_def.define("build_neighbor_list_cell_list_out(Tensor coords, Tensor? box, Scalar cutoff, Tensor(a!) pairs, "
-            "Tensor? excl_row_ptr, Tensor? excl_col_indices, bool include_self) -> (Tensor, Tensor)")
-
-
-def _nblist_into(coords, box, cutoff, pairs, excl_row_ptr, excl_col, include_self, algorithm):
-    L.require_cuda(coords, box, excl_row_ptr, excl_col, pairs)
-    lib = L.load()
-    coords = coords.detach().contiguous()
-    n = coords.shape[0]
-    if pairs.dtype != torch.int64 or pairs.dim() != 2 or pairs.shape[1] != 2 or not pairs.is_contiguous():
-        raise RuntimeError("pairs must be a contiguous (max_npairs, 2) int64 tensor")
-    if algorithm == L.NBLIST_CELL_LIST and box is None:
-        raise RuntimeError("Cell-list neighbor list requires a periodic box")
-    bh = L.box_host(box)
-    if excl_row_ptr is not None:
-        excl_row_ptr = excl_row_ptr.to(torch.int64).contiguous()
-        excl_col = excl_col.to(torch.int64).contiguous()
-        if excl_row_ptr.numel() != n + 1:
-            raise RuntimeError(f"excl_row_ptr must have natoms + 1 = {n + 1} entries")
-    npairs = torch.empty(1, dtype=torch.int32, device=coords.device)
-    nbytes = lib.tff_nblist_workspace_bytes(n, bh, float(cutoff), algorithm)
-    ws = L.workspace(nbytes, coords.device)
-    rc = lib.tff_build_neighbor_list(L.dtype_code(coords), L.ptr(coords), n, bh, float(cutoff), algorithm,
-                                     int(bool(include_self)), L.ptr(excl_row_ptr), L.ptr(excl_col), L.ptr(pairs),
-                                     pairs.shape[0], L.ptr(npairs), L.ptr(ws), ws.numel(), L.stream_of(coords))
-    L.check(rc, "build_neighbor_list")
-    return npairs
-
-
-def _capacity(n, max_npairs):
-    m = int(max_npairs)
-    return n * (n + 1) // 2 if m < 0 else m
-
-
-def _nsq(coords, box, cutoff, max_npairs, excl_row_ptr, excl_col_indices, include_self):
-    pairs = torch.empty((_capacity(coords.shape[0], max_npairs), 2), dtype=torch.int64, device=coords.device)
-    npairs = _nblist_into(coords, box, cutoff, pairs, excl_row_ptr, excl_col_indices, include_self, L.NBLIST_NSQUARED)
-    return pairs, npairs
-
-
-def _nsq_out(coords, box, cutoff, pairs, excl_row_ptr, excl_col_indices, include_self):
-    return _nblist_into(coords, box, cutoff, pairs, excl_row_ptr, excl_col_indices, include_self, L.NBLIST_NSQUARED)
-
-
-def _cl(coords, box, cutoff, max_npairs, excl_row_ptr, excl_col_indices, include_self):
-    if box is None:
-        raise RuntimeError("Cell-list neighbor list requires a periodic box")
-    pairs = torch.empty((_capacity(coords.shape[0], max_npairs), 2), dtype=torch.int64, device=coords.device)
-    npairs = _nblist_into(coords, box, cutoff, pairs, excl_row_ptr, excl_col_indices, include_self, L.NBLIST_CELL_LIST)
-    return pairs, npairs
-
-
-def _cl_out(coords, box, cutoff, pairs, excl_row_ptr, excl_col_indices, include_self):
-    npairs = _nblist_into(coords, box, cutoff, pairs, excl_row_ptr, excl_col_indices, include_self, L.NBLIST_CELL_LIST)
-    return pairs, npairs
-
-
-_cuda.impl("build_neighbor_list_nsquared", _nsq)
-_cuda.impl("build_neighbor_list_nsquared_out", _nsq_out)
-_cuda.impl("build_neighbor_list_cell_list", _cl)
-_cuda.impl("build_neighbor_list_cell_list_out", _cl_out)
-
-
-# ---------------------------------------------------------------- pair terms
-def _pair_call(mode, coords, pairs, box, charges=None, sigma=None, epsilon=None, atom_types=None,
-               pairs_excl=None, coul=0.0, cutoff=0.0, alpha=-1.0, do_shift=True, want=(), forces=None,
-               energy_shape=(1,)):
-    """Run one pair-term kernel; returns (energy, {name: grad}) for the names in ``want``."""
-    L.require_cuda(coords, pairs, box, charges, sigma, epsilon, atom_types, pairs_excl, forces)
-    lib = L.load()
-    dt = coords.dtype
-    coords_c = coords.detach().contiguous()
-    n = coords_c.shape[0]
-    pairs = pairs.detach().to(torch.int64).contiguous()
-    P = pairs.shape[0]
-    pe = None if pairs_excl is None else pairs_excl.detach().to(torch.int64).contiguous()
-    PE = 0 if pe is None else pe.shape[0]
-
-    def prep(x):
-        return None if x is None else x.detach().to(dt).contiguous()
-
-    q, sg, ep = prep(charges), prep(sigma), prep(epsilon)
-    ty = None if atom_types is None else atom_types.detach().to(torch.int64).contiguous()
-    ntypes = 0 if (ty is None or sg is None) else sg.shape[0]
-    energy = torch.empty(energy_shape, dtype=dt, device=coords.device)
-    grads = {}
-    for name, ref in (("coords", coords_c), ("charges", q), ("sigma", sg), ("epsilon", ep)):
-        if name in want and ref is not None:
-            grads[name] = torch.empty_like(ref)
-    nbytes = lib.tff_pair_workspace_bytes(mode, n, P, ntypes)
-    ws = L.workspace(nbytes, coords.device)
-    rc = lib.tff_pair_energy(L.dtype_code(coords_c), mode, L.ptr(coords_c), n, L.box_host(box), L.ptr(pairs), P,
-                             L.ptr(pe), PE, L.ptr(q), L.ptr(sg), L.ptr(ep), L.ptr(ty), ntypes, float(coul),
-                             float(cutoff), float(alpha), int(bool(do_shift)), 1.0, L.ptr(energy),
-                             L.ptr(grads.get("coords")), L.ptr(grads.get("charges")), L.ptr(grads.get("sigma")),
-                             L.ptr(grads.get("epsilon")), L.ptr(forces), L.ptr(ws), ws.numel(),
-                             L.stream_of(coords_c))
-    L.check(rc, "pair energy")
-    return energy, grads
-
-
-class _PairEnergy(torch.autograd.Function):
-    """Energy op whose forward also produces the input gradients (reference design)."""
-
-    @staticmethod
-    def forward(ctx, mode, coords, pairs, box, charges, sigma, epsilon, atom_types, pairs_excl, coul, cutoff,
-                alpha, do_shift, energy_shape):
-        want = [k for k, t in (("coords", coords), ("charges", charges), ("sigma", sigma), ("epsilon", epsilon))
-                if t is not None and t.requires_grad]
-        e, g = _pair_call(mode, coords, pairs, box, charges, sigma, epsilon, atom_types, pairs_excl, coul, cutoff,
-                          alpha, do_shift, want=want, energy_shape=energy_shape)
-        ctx.names = list(g.keys())
-        ctx.save_for_backward(*[g[k] for k in ctx.names])
-        return e
-
-    @staticmethod
-    def backward(ctx, ge):
-        saved = dict(zip(ctx.names, ctx.saved_tensors))
-        s = ge.reshape(())
-
-        def gr(name):
-            return saved[name] * s if name in saved else None
-
-        return (None, gr("coords"), None, None, gr("charges"), gr("sigma"), gr("epsilon"), None, None, None, None,
-                None, None, None)
-
-
-_def.define("compute_nonbonded_energy_from_atom_pairs(Tensor coords, Tensor pairs, Tensor box, Tensor sigma, "
-            "Tensor epsilon, Tensor charges, Scalar coul_constant, Scalar cutoff, bool do_shift) -> Tensor")
-_def.define("compute_nonbonded_forces_from_atom_pairs(Tensor coords, Tensor pairs, Tensor box, Tensor sigma, "
-            "Tensor epsilon, Tensor charges, Scalar coul_constant, Scalar cutoff, Tensor(a!) forces) -> ()")
-_def.define("compute_nonbonded_ewald_energy_from_atom_pairs(Tensor coords, Tensor pairs, Tensor box, "
-            "Tensor sigma, Tensor epsilon, Tensor charges, Scalar coul_constant, Scalar cutoff, "
-            "Scalar ewald_alpha, Tensor? pairs_excl) -> Tensor")
-_def.define("compute_multipolar_energy_from_atom_pairs(Tensor coords, Tensor box, Tensor pairs, "
-            "Tensor? pairs_excl, Tensor q, Tensor? p, Tensor? t, Scalar cutoff, Scalar ewald_alpha, "
-            "Scalar prefactor) -> Tensor")
-_def.define("compute_lennard_jones_energy(Tensor coords, Tensor pairs, Tensor box, Tensor sigma, "
-            "Tensor epsilon, Scalar cutoff, Tensor? atom_types) -> Tensor")
-
-
-def _nonbonded_energy(coords, pairs, box, sigma, epsilon, charges, coul_constant, cutoff, do_shift):
-    return _PairEnergy.apply(L.PAIR_NONBONDED, coords, pairs, box, charges, sigma, epsilon, None, None,
-                             float(coul_constant), float(cutoff), -1.0, bool(do_shift), (1,))
-
-
-def _nonbonded_forces(coords, pairs, box, sigma, epsilon, charges, coul_constant, cutoff, forces):
-    if forces.dtype != coords.dtype or not forces.is_contiguous() or forces.shape != coords.shape:
-        raise RuntimeError("forces must be a contiguous tensor shaped and typed like coords")
-    _pair_call(L.PAIR_NONBONDED, coords, pairs, box, charges, sigma, epsilon, coul=float(coul_constant),
-               cutoff=float(cutoff), do_shift=True, forces=forces)
-
-
-def _nonbonded_ewald_energy(coords, pairs, box, sigma, epsilon, charges, coul_constant, cutoff, ewald_alpha,
-                            pairs_excl):
-    return _PairEnergy.apply(L.PAIR_EWALD_LJ, coords, pairs, box, charges, sigma, epsilon, None, pairs_excl,
-                             float(coul_constant), float(cutoff), float(ewald_alpha), False, (1,))
-
-
-def _multipolar_energy(coords, box, pairs, pairs_excl, q, p, t, cutoff, ewald_alpha, prefactor):
-    if p is not None or t is not None:
-        raise NotImplementedError("compute_multipolar_energy_from_atom_pairs: only rank 0 (point charges) is "
-                                  "provided by the gfx950 build")
-    return _PairEnergy.apply(L.PAIR_COUL_EWALD, coords, pairs, box, q, None, None, None, pairs_excl,
-                             float(prefactor), float(cutoff), float(ewald_alpha), False, ())
-
-
-def _lj_energy(coords, pairs, box, sigma, epsilon, cutoff, atom_types):
-    mode = L.PAIR_LJ_TYPES if atom_types is not None else L.PAIR_LJ_PAIRS
-    return _PairEnergy.apply(mode, coords, pairs, box, None, sigma, epsilon, atom_types, None, 0.0, float(cutoff),
-                             -1.0, False, ())
-
-
-_auto.impl("compute_nonbonded_energy_from_atom_pairs", _nonbonded_energy)
-_cuda.impl("compute_nonbonded_forces_from_atom_pairs", _nonbonded_forces)
-_auto.impl("compute_nonbonded_ewald_energy_from_atom_pairs", _nonbonded_ewald_energy)
-_auto.impl("compute_multipolar_energy_from_atom_pairs", _multipolar_energy)
-_auto.impl("compute_lennard_jones_energy", _lj_energy)
-
 
 # ---------------------------------------------------------------- cluster pairs
 _def.define("build_cluster_pairs(Tensor coords, Tensor box, Scalar cutoff, Tensor? exclusions, Scalar cell_size, "
@@ -413,8 +240,9 @@ _cuda.impl("compute_nonbonded_forces_from_cluster_pairs", _cluster_forces)
 
 
 # ---------------------------------------------------------------- PME
-_def.define("pme_long_range(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar K, Scalar alpha, "
-            "Tensor xmoduli, Tensor ymoduli, Tensor zmoduli, int order=6) -> Tensor")
+# pme_long_range itself is registered in C++; point dipoles and quadrupoles are forwarded to this op
+_def.define("_pme_long_range_multipole(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar K, "
+            "Scalar alpha, Tensor xmoduli, Tensor ymoduli, Tensor zmoduli, int order=6) -> Tensor")
 
 
 def _grid_dims(K):
@@ -452,33 +280,6 @@ def _pme_forward(coords, box, q, K, alpha, xm, ym, zm, order):
                             L.ptr(ws), ws.numel(), L.stream_of(c))
     L.check(rc, "pme_long_range")
     return energy, c, qq, phi, (K0, K1, K2, int(order), float(alpha))
-
-
-class _PmeEnergy(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, coords, box, q, K, alpha, xm, ym, zm, order):
-        energy, c, qq, phi, meta = _pme_forward(coords, box, q, K, alpha, xm, ym, zm, order)
-        ctx.save_for_backward(c, qq, phi)
-        ctx.box = box
-        ctx.meta = meta
-        ctx.need = (coords.requires_grad, q.requires_grad)
-        return energy
-
-    @staticmethod
-    def backward(ctx, ge):
-        c, qq, phi = ctx.saved_tensors
-        K0, K1, K2, order, alpha = ctx.meta
-        lib = L.load()
-        n = c.shape[0]
-        dc = torch.empty_like(c) if ctx.need[0] else None
-        dq = torch.empty_like(qq) if ctx.need[1] else None
-        rc = lib.tff_pme_gather(L.dtype_code(c), L.ptr(phi), L.ptr(c), L.ptr(qq), n, L.box_host(ctx.box), K0, K1, K2,
-                                order, alpha, 1.0, L.ptr(dq), L.ptr(dc), L.stream_of(c))
-        L.check(rc, "pme_long_range backward")
-        # scale by the incoming gradient on the device (no host synchronisation)
-        s = ge.reshape(())
-        return (None if dc is None else dc * s), None, (None if dq is None else dq * s), None, None, None, None, \
-            None, None
 
 
 # ---- multipoles (ranks 1 and 2)
@@ -598,13 +399,11 @@ def _pme_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, or
     return _PmeMultipole.apply(coords, box, q, p, t9, int(K), float(alpha), xmoduli, ymoduli, zmoduli, int(order))
 
 
-def _pme_long_range(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order=6):
-    if p is not None or t is not None:
-        return _pme_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order)[0]
-    return _PmeEnergy.apply(coords, box, q, int(K), float(alpha), xmoduli, ymoduli, zmoduli, int(order))
+def _pme_long_range_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order=6):
+    return _pme_multipole(coords, box, q, p, t, K, alpha, xmoduli, ymoduli, zmoduli, order)[0]
 
 
-_auto.impl("pme_long_range", _pme_long_range)
+_auto.impl("_pme_long_range_multipole", _pme_long_range_multipole)
 
 _def.define("pme_long_range_all(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar K, Scalar alpha, "
             "Tensor xmoduli, Tensor ymoduli, Tensor zmoduli, int order=6) -> (Tensor, Tensor, Tensor)")
