@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--workload", default="water96k")
     ap.add_argument("--dropin-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-samples", type=int, default=3, help="timed CPU oracle evaluations (median reported)")
     ap.add_argument("--only", default="", help="comma list of paths to run (mixed,fp64,dropin)")
     ap.add_argument("--differentiation", action="store_true",
                     help="time force-matching steps (E, F, loss, gradients wrt coords, q, sigma, eps) instead")
@@ -88,22 +89,39 @@ def calibrate_copy(mb, steps):
                       "ms": sec * 1e3, "GB/s": 8 * n / sec / 1e9}))
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "fused_tiles_traffic_water96k.md")
+KERNEL_SOURCE = os.path.join("torchff-lib_amd", "csrc", "fused.hip")
+
+
+def source_sha256(rel):
+    import hashlib
+    with open(os.path.join(ROOT, rel), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def tile_traffic():
     """HBM bytes per launch of the pair kernel from the committed counter summary (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "fused_tiles_traffic_water96k.md")
+    WRITE_SIZE, scaled by the copy-kernel calibration), and where it came from.  The summary
+    records the sha256 of the kernel source it was measured on; a summary of another source
+    gives traffic None (stale)."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    info = {"profile": TRAFFIC_PROFILE, "kernel_source_sha256": source_sha256(KERNEL_SOURCE)}
     if not os.path.exists(path):
-        return None
+        return None, info
     vals = {}
     with open(path) as f:
         for line in f:
             line = line.strip().lstrip("- ")
-            if line.startswith(("fetch_bytes_per_launch:", "write_bytes_per_launch:")):
+            if line.startswith(("fetch_bytes_per_launch:", "write_bytes_per_launch:", "fused_hip_sha256:")):
                 k, v = line.split(":", 1)
-                vals[k.strip()] = float(v.strip().split()[0])
-    if len(vals) != 2:
-        return None
-    return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"]
+                vals[k.strip()] = v.strip().split()[0]
+    info["measured_on_this_source"] = vals.get("fused_hip_sha256") == info["kernel_source_sha256"]
+    if not info["measured_on_this_source"] or "fetch_bytes_per_launch" not in vals \
+            or "write_bytes_per_launch" not in vals:
+        return None, info
+    info["fetch_bytes"] = float(vals["fetch_bytes_per_launch"])
+    info["write_bytes"] = float(vals["write_bytes_per_launch"])
+    return info["fetch_bytes"] + info["write_bytes"], info
 
 
 def pme_stage_bytes(w):
@@ -518,8 +536,10 @@ def main():
     if head == "mixed":
         t = kern["fused_tiles"]["avg_ms"]
         achieved = p_in * PAIR_FLOPS / (t * 1e-3) / 1e12
+        traffic, traffic_info = tile_traffic()
         roof = {"kernel": "fused_tiles", "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": tile_traffic(), "avg_ms": t,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                "traffic_source": traffic_info, "avg_ms": t,
                 "units": f"{p_in} in-cutoff pairs x {PAIR_FLOPS} FLOP per launch",
                 "tiles": tiles, "pair_slot_efficiency": p_in / (tiles * 1024.0)}
     na = w["natoms"]
@@ -586,14 +606,21 @@ def main():
         from oracle import composite
         threads = os.cpu_count() or 1
         torch.set_num_threads(threads)
-        t0 = time.perf_counter()
+        # one warm-up evaluation (also the error reference), then the median of cpu_samples
         E_ref, F_ref, terms, _ = composite.energy_forces(w)
-        dt = time.perf_counter() - t0
+        times = []
+        for _ in range(args.cpu_samples):
+            t0 = time.perf_counter()
+            composite.energy_forces(w)
+            times.append(time.perf_counter() - t0)
+        dt = float(np.median(times))
         out["cpu_baseline"] = {
             "value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"one full float64 energy+force evaluation of the same {w['natoms']}-atom box by the CPU "
-                      f"oracle (numpy cell search + torch autograd real space + PME), {dt:.1f} s on {threads} "
-                      f"threads of {platform.processor() or platform.machine()}"}
+            "samples": len(times), "sample_seconds": times,
+            "sample": f"median of {len(times)} full float64 energy+force evaluations (after one warm-up) of the "
+                      f"same {w['natoms']}-atom box by the CPU oracle (numpy cell search + torch autograd real "
+                      f"space + PME), {dt:.1f} s each on {threads} threads of "
+                      f"{platform.processor() or platform.machine()}"}
         scale = sum(abs(v) for v in terms.values())
         out["errors_vs_fp64_oracle"] = {
             k: {"energy_abs": abs(v[0] - E_ref), "energy_rel_to_sum_abs_terms": abs(v[0] - E_ref) / scale,
