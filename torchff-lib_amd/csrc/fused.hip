@@ -716,6 +716,7 @@ struct TileArgs {
   TC rc2;
   TC alpha;
   TC coul;
+  TC c2alpha;  // (2 / sqrt(pi)) alpha
 };
 
 // Up to 16 sub-tiles of 8 x 8 atoms of one cluster-pair tile.  Lane (ii = lane & 7, jj = lane >> 3)
@@ -727,7 +728,6 @@ __device__ __forceinline__ void tile_steps(unsigned mask, bool upper, const V4<T
                                            const V4<TC> (&pj)[4], const V2<TC> (&lJ)[4], const unsigned (&exm)[4],
                                            int jj, const TileArgs<TC>& ta, TC (&fi)[4][3], TC (&fj)[4][3], TC& e_c,
                                            TC& e_lj) {
-  const TC c2 = (TC)1.1283791670955126;  // 2 / sqrt(pi)
   // j-group major, so that the first steps wait only for the first j-group's loads
 #pragma unroll
   for (int b = 0; b < 16; ++b) {
@@ -755,22 +755,21 @@ __device__ __forceinline__ void tile_steps(unsigned mask, bool upper, const V4<T
     r2 = ok ? r2 : (TC)1e30;
     const TC rinv = rsqrt_t(r2);
     const TC rinv2 = rinv * rinv;
-    const TC u = ta.alpha * (r2 * rinv);
+    const TC r = r2 * rinv;
     TC ec, em;
-    erfc_exp(u, ec, em);
+    erfc_exp(ta.alpha * r, ec, em);
     if (EXCL) ec = excl ? ec - (TC)1 : ec;
     const TC qr = pi[gi].w * pj[gj].w * rinv;
-    const TC ecr = qr * ec;
-    TC fs = qr * fma(c2 * u, em, ec) * rinv2;
     TC ep4 = li[gi].y * lJ[gj].y;
     if (EXCL) ep4 = excl ? (TC)0 : ep4;
     const TC sg = li[gi].x + lJ[gj].x;
     const TC s2 = sg * sg * rinv2;
     const TC s6 = s2 * s2 * s2;
-    const TC elj = ep4 * fma(s6, s6, -s6);
-    fs = fma(ep4 * s6 * fma((TC)12, s6, (TC)-6), rinv2, fs);
-    e_c += ecr;
-    e_lj += elj;
+    const TC a6 = ep4 * s6;  // 4 eps s^6
+    // -dE/dr / r = [k q_i q_j / r (erfc + (2/sqrt pi) a r e^{-a^2 r^2}) + 4 eps s^6 (12 s^6 - 6)] / r^2
+    const TC fs = fma(qr, fma(ta.c2alpha * r, em, ec), a6 * fma((TC)12, s6, (TC)-6)) * rinv2;
+    e_c = fma(qr, ec, e_c);
+    e_lj += fma(a6, s6, -a6);
     // F_i += fs d, F_j -= fs d   (fs = -dE/dr / r)
     fi[gi][0] = fma(fs, dx, fi[gi][0]);
     fi[gi][1] = fma(fs, dy, fi[gi][1]);
@@ -1990,6 +1989,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     ta.rc2 = (TC)(cutoff * cutoff);
     ta.alpha = (TC)alpha;
     ta.coul = (TC)coul;
+    ta.c2alpha = (TC)(1.1283791670955126 * alpha);
     void* tok = tff_prof_begin("fused_tiles", st);
     if constexpr (sizeof(TC) == 4)
       hipLaunchKernelGGL(k_tiles_f32, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
@@ -2068,6 +2068,7 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
     ta.rc2 = (TC)(cutoff * cutoff);
     ta.alpha = (TC)alpha;
     ta.coul = (TC)coul;
+    ta.c2alpha = (TC)(1.1283791670955126 * alpha);
     void* tok = tff_prof_begin("fused_grad_tiles", st);
     hipLaunchKernelGGL(k_tiles_grad<TC>, dim3((unsigned)w.nw_tiles), dim3(TFF_WAVE), 0, st, w.cpairs, w.ncp,
                        g.pair_cap, (const V4<TC>*)w.xyzq, (const V2<TC>*)w.lj, (const V4<TC>*)w.vsl, w.excl_part, ta,

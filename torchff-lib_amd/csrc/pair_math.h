@@ -4,20 +4,16 @@
 #include <hip/hip_runtime.h>
 
 // erfc(u) and exp(-u^2) for u >= 0.
-// float: exp(-u^2) = 2^(-u^2 log2 e) with the product split into a rounded part and its fma
-// remainder (plus the rounding of log2 e in float), the rounded part through the hardware
-// base-2 exponential and the remainder as a first-order factor; erfc(u) exp(u^2) =
-// sum_k c_k t^k, t = 1 / (1 + 0.39 u) (hardware reciprocal), a degree-8 polynomial fitted
-// to the scaled complementary error function on [0, 3.6] by iteratively reweighted least
-// squares (relative error 1.3e-9 in exact arithmetic, about 1.5e-6 evaluated in float32,
-// against scipy.special.erfc).  double: the correctly rounded library functions.
+// float: exp(-u^2) = 2^(-u^2 log2 e) through the hardware base-2 exponential (relative error
+// below 7e-7 for u <= 3 from the rounding of the argument); erfc(u) exp(u^2) = sum_k c_k t^k,
+// t = 1 / (1 + 0.39 u) (hardware reciprocal), a degree-8 polynomial fitted to the scaled
+// complementary error function on [0, 3.6] by iteratively reweighted least squares (relative
+// error 1.3e-9 in exact arithmetic, about 1.5e-6 evaluated in float32, against
+// scipy.special.erfc).  The fit is kept this accurate because rigid molecules put every
+// intramolecular pair at the same u, where a polynomial error would add up coherently over the
+// exclusion corrections.  double: the correctly rounded library functions.
 __device__ __forceinline__ void erfc_exp(float u, float& ec, float& em) {
-  const float v = u * u;
-  const float hi = v * -1.44269502f;
-  float lo = fmaf(v, -1.44269502f, -hi);
-  lo = fmaf(v, -1.92596299e-8f, lo);
-  const float e = __builtin_amdgcn_exp2f(hi);
-  em = fmaf(e, lo * 0.693147181f, e);
+  em = __builtin_amdgcn_exp2f(u * u * -1.44269502f);
   const float t = __builtin_amdgcn_rcpf(fmaf(0.39f, u, 1.0f));
   float p = 9.27117256e-02f;
   p = fmaf(p, t, -5.29840954e-01f);
@@ -34,10 +30,6 @@ __device__ __forceinline__ void erfc_exp(double u, double& ec, double& em) {
   em = exp(-u * u);
   ec = erfc(u);
 }
-// hardware reciprocal square root refined by one Newton step (Lennard-Jones raises its error
-// to the sixth power)
-__device__ __forceinline__ float rsqrt_t(float x) {
-  const float y = __builtin_amdgcn_rsqf(x);
-  return y * fmaf(-0.5f * x, y * y, 1.5f);
-}
+// hardware reciprocal square root (1 ulp; Lennard-Jones raises the error to 12 ulp of s^12)
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
