@@ -59,6 +59,8 @@ def parse():
                     help="time force-matching steps (E, F, loss, gradients wrt coords, q, sigma, eps) instead")
     ap.add_argument("--widened", action="store_true",
                     help="multipolar PME (ranks 0-2) and the direct Ewald op, energy + gradients per call")
+    ap.add_argument("--md", action="store_true",
+                    help="only time the torch-sim style consumer (16 boxes per call against one call per box)")
     ap.add_argument("--serving", action="store_true",
                     help="serve 4096 Poisson requests (20-1000 waters) batched and one system per launch instead")
     ap.add_argument("--requests", type=int, default=4096)
@@ -295,34 +297,7 @@ def widened_bench(args, w, dev):
         m = torchff.Ewald(w9["alpha"], 10, rank, return_fields=False)
         sec, _ = run(m, *inputs(w9, torch.float64, rank, 10 + rank), 5, None)
         res[f"ewald_rank{rank}_f64_water900_kmax10"] = {"ms": sec * 1e3, "evals_per_s": 1.0 / sec}
-    # the torch-sim style consumer: 16 water boxes of 20-300 waters in one state per call, against
-    # the same boxes one call each (mixed precision serving engine)
-    from types import SimpleNamespace
-    from torchff.md import BatchedSimModel, NM_PER_ANGSTROM
-    from torchff.serving import ServingEngine
-    from torchff.workloads import box_length_for_density, water_box
-    rng = np.random.default_rng(5)
-    sizes = [int(x) for x in np.exp(rng.uniform(np.log(20), np.log(300), 16))]
-    ws = [water_box(nw, box_length_for_density(nw), seed=100 + k) for k, nw in enumerate(sizes)]
-    eng = ServingEngine(slots=2, max_systems=16, max_atoms=16384, max_grid_points=1 << 18, max_k=32,
-                        precision="mixed", device=dev)
-
-    def state_of(sel):
-        P = np.concatenate([ws[k]["coords"] for k in sel]) / NM_PER_ANGSTROM
-        idx = np.concatenate([np.full(ws[k]["natoms"], j) for j, k in enumerate(sel)])
-        cells = np.stack([np.asarray(ws[k]["box"]).T / NM_PER_ANGSTROM for k in sel])
-        par = [np.concatenate([np.asarray(ws[k][f]) for k in sel]) for f in ("charges", "sigma", "epsilon")]
-        mol = np.concatenate([np.arange(ws[k]["natoms"]) // 3 for k in sel])
-        st = SimpleNamespace(positions=torch.tensor(P, device=dev), cell=torch.tensor(cells, device=dev),
-                             system_idx=torch.tensor(idx, device=dev))
-        return st, BatchedSimModel(eng, *par, mol)
-
-    st_all, m_all = state_of(list(range(16)))
-    singles = [state_of([k]) for k in range(16)]
-    sec_all, _ = time_steps(lambda: m_all(st_all), 20, 3)
-    sec_one, _ = time_steps(lambda: [m(st) for st, m in singles], 5, 1)
-    res["md_batched_16_systems"] = {"ms": sec_all * 1e3, "systems": 16, "waters": sizes,
-                                    "one_call_per_system_ms": sec_one * 1e3}
+    res.update(md_consumer_results(dev))
     head = res["pme_rank2_f32"]
     print(json.dumps({
         "metric": f"PME energy + gradient evals/s, point charges + dipoles + quadrupoles, {w['natoms'] // 1000}k "
@@ -335,6 +310,43 @@ def widened_bench(args, w, dev):
                                "16 water boxes through the torch-sim style model",
                    "path": "torchff.PME(rank) / torchff.Ewald(rank) drop-in ops, autograd backward"},
         "results": res}))
+
+
+def md_consumer_results(dev):
+    """The torch-sim style consumer: 16 water boxes of 20-300 waters in one state per call (the
+    batch resident on the device, positions written in place each step), against the same boxes
+    one model call each (mixed precision serving engine)."""
+    from types import SimpleNamespace
+    from torchff.md import BatchedSimModel, NM_PER_ANGSTROM
+    from torchff.serving import ServingEngine
+    from torchff.workloads import box_length_for_density, water_box
+    res = {}
+    rng = np.random.default_rng(5)
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(20), np.log(300), 16))]
+    ws = [water_box(nw, box_length_for_density(nw), seed=100 + k) for k, nw in enumerate(sizes)]
+    eng = ServingEngine(slots=2, max_systems=16, max_atoms=16384, max_grid_points=1 << 18, max_k=32,
+                        precision="mixed", device=dev)
+
+    def state_of(sel, engine=eng):
+        P = np.concatenate([ws[k]["coords"] for k in sel]) / NM_PER_ANGSTROM
+        idx = np.concatenate([np.full(ws[k]["natoms"], j) for j, k in enumerate(sel)])
+        cells = np.stack([np.asarray(ws[k]["box"]).T / NM_PER_ANGSTROM for k in sel])
+        par = [np.concatenate([np.asarray(ws[k][f]) for k in sel]) for f in ("charges", "sigma", "epsilon")]
+        mol = np.concatenate([np.arange(ws[k]["natoms"]) // 3 for k in sel])
+        st = SimpleNamespace(positions=torch.tensor(P, device=dev), cell=torch.tensor(cells, device=dev),
+                             system_idx=torch.tensor(idx, device=dev))
+        return st, BatchedSimModel(engine, *par, mol)
+
+    st_all, m_all = state_of(list(range(16)))
+    # one small engine per single-system model (each keeps its batch resident in its own slot)
+    singles = [state_of([k], ServingEngine(slots=1, max_systems=1, max_atoms=1024, max_grid_points=1 << 15,
+                                           max_k=32, precision="mixed", device=dev)) for k in range(16)]
+    sec_all, _ = time_steps(lambda: m_all(st_all), 20, 3)
+    sec_req, _ = time_steps(lambda: m_all.forward_via_requests(st_all), 10, 2)
+    sec_one, _ = time_steps(lambda: [m(st) for st, m in singles], 5, 1)
+    res["md_batched_16_systems"] = {"ms": sec_all * 1e3, "systems": 16, "waters": sizes,
+                                    "request_path_ms": sec_req * 1e3, "one_call_per_system_ms": sec_one * 1e3}
+    return res
 
 
 def serving_results(args, dev):
@@ -506,6 +518,9 @@ def main():
         return
     if args.widened:
         widened_bench(args, w, dev)
+        return
+    if args.md:
+        print(json.dumps({"md_consumer": md_consumer_results(dev)}))
         return
     res = {}
     outs = {}

@@ -342,7 +342,10 @@ int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void
  * table order [real space (LJ + Coulomb incl. the exclusion correction), 0, reciprocal, self,
  * total] at out[5 s], then from out[5 max_systems] the forces -dE/dr (3 per atom record, rows of
  * records in no system not written).  A new workspace is zeroed once by
- * tff_serve_workspace_init; the charge grid is cleaned by each evaluation for the next. */
+ * tff_serve_workspace_init; the charge grid is cleaned by each evaluation for the next.  The
+ * workspace begins with the table's layout: after tff_serve_upload the atom records sit at byte
+ * tff_serve_atoms_offset(max_systems) of the workspace, where the caller may rewrite their
+ * coordinates on the device between evaluations (e.g. each MD step) without another upload. */
 #define TFF_FULL 5 /* tff_serve_table_add: the system does not fit the capacities (table unchanged) */
 
 size_t tff_serve_table_bytes(int max_systems, int max_atoms);

@@ -87,3 +87,39 @@ def test_each_system_alone_equals_the_batch():
         assert torch.equal(alone["energy"][0], both["energy"][k])
         assert torch.equal(alone["forces"], both["forces"][sl])
         start += w["natoms"]
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_resident_steps_match_the_request_path(precision):
+    # positions move every step (device only); then the cell tensor is replaced (re-upload)
+    ws = _systems([25, 80, 140], [11, 12, 13])
+    n = sum(w["natoms"] for w in ws)
+    order = np.argsort(np.arange(n) % 3, kind="stable")
+    st, par, mol = _state(ws, order)
+    model = BatchedSimModel(_engine(precision), par["charges"], par["sigma"], par["epsilon"], mol)
+    ref_model = BatchedSimModel(_engine(precision), par["charges"], par["sigma"], par["epsilon"], mol)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    for step in range(4):
+        if step == 3:
+            st.cell = st.cell * 1.01  # a new cell tensor: the batch is rebuilt
+        out = model(st)
+        ref = ref_model.forward_via_requests(st)
+        assert torch.equal(out["energy"], ref["energy"]), step
+        assert torch.equal(out["forces"], ref["forces"]), step
+        st.positions = st.positions + 0.01 * torch.randn(st.positions.shape, device="cuda", dtype=st.positions.dtype,
+                                                         generator=gen)
+
+
+def test_resident_model_keeps_positions_on_the_device():
+    # after the first call (the upload), steps read no positions back to the host
+    ws = _systems([40, 70], [21, 22])
+    st, par, mol = _state(ws)
+    model = BatchedSimModel(_engine("mixed"), par["charges"], par["sigma"], par["epsilon"], mol)
+    model(st)
+    key = model._res["key"]
+    from unittest import mock
+    with mock.patch("torchff.md.state_requests", side_effect=AssertionError("host path taken")):
+        for _ in range(3):
+            st.positions = st.positions + 0.001
+            model(st)
+    assert model._res["key"][3] == key[3]  # same cell: no rebuild

@@ -68,7 +68,17 @@ class BatchedSimModel(nn.Module):
     """torch-sim style model over the serving engine (see the module docstring).
 
     ``charges``, ``sigma``, ``epsilon``, ``molecule``: per atom of the states this model will see
-    (the same atom order); ``engine``: a ``ServingEngine`` whose capacities fit the batch."""
+    (the same atom order); ``engine``: a ``ServingEngine`` whose capacities fit the batch.
+
+    The batch stays resident on the device in the engine's first slot: the host builds and
+    uploads its table only when the batch composition changes (another atom count, system
+    index tensor or cell tensor, or the slot used for other work in between).  Each call then
+    writes the state's positions into the slot's device atom records, replays the captured
+    evaluation on the slot's stream and gathers energies and forces on the device: no
+    device-to-host copy of positions and no host synchronisation per step.  A cell or system
+    index changed in place without a version bump (through ``.data`` or a view) is not seen;
+    give the state a new tensor (or call :meth:`invalidate`).  Calls on one engine from several
+    models take turns on that slot (each change of owner re-uploads the table)."""
 
     def __init__(self, engine: ServingEngine, charges, sigma, epsilon, molecule, spacing: float = 0.1,
                  device: Optional[torch.device] = None, dtype: torch.dtype = torch.float64):
@@ -81,6 +91,7 @@ class BatchedSimModel(nn.Module):
         self.spacing = float(spacing)
         self._device = torch.device(device) if device is not None else engine.device
         self._dtype = dtype
+        self._res = None
 
     # the torch-sim ModelInterface properties
     @property
@@ -99,7 +110,78 @@ class BatchedSimModel(nn.Module):
     def compute_forces(self) -> bool:
         return True
 
+    def invalidate(self):
+        """Forget the resident batch: the next call rebuilds and uploads the table."""
+        self._res = None
+
+    @staticmethod
+    def _tensor_key(t):
+        return None if t is None else (t.data_ptr(), t._version, tuple(t.shape), t.device)
+
+    def _key(self, state):
+        idx = getattr(state, "system_idx", None)
+        if idx is None:
+            idx = getattr(state, "batch", None)
+        pos = state.positions
+        return (id(self), tuple(pos.shape), pos.device, self._tensor_key(torch.as_tensor(state.cell)),
+                self._tensor_key(idx if isinstance(idx, torch.Tensor) else None),
+                None if isinstance(idx, torch.Tensor) or idx is None else tuple(np.asarray(idx).reshape(-1)))
+
+    def _setup(self, state, key):
+        """Host side, once per batch composition: one request per system, the table built in the
+        slot and uploaded, and the maps between state order and the slot's atom records
+        (systems in order, each system's atoms in state order)."""
+        slot = self.engine.slots[0]
+        if slot.requests:
+            raise RuntimeError("BatchedSimModel needs an idle engine slot")
+        reqs = state_requests(state, self.charges, self.sigma, self.epsilon, self.molecule, self.spacing)
+        slot.reset()
+        for r in reqs:
+            if not slot.try_add(r):
+                slot.reset()
+                raise RuntimeError(f"system {r.rid} ({r.natoms} atoms, grid {r.grid}) does not fit the engine's "
+                                   "capacities next to the others")
+        n = sum(r.natoms for r in reqs)
+        with torch.cuda.stream(slot.stream):
+            slot.upload()
+        slot.stream.synchronize()  # the pinned host table is reused by the next upload
+        slot.reset()
+        sys_idx = _system_index(state, n)
+        order = np.concatenate([np.nonzero(sys_idx == r.rid)[0] for r in reqs]).astype(np.int64)
+        dev = slot.ws.device
+        perm = torch.as_tensor(order, device=dev)
+        inv = torch.empty_like(perm)
+        inv[perm] = torch.arange(n, device=dev)
+        slot.resident_key = key
+        # a device divisor: torch divides by a host scalar through its reciprocal, the request path
+        # (numpy) by true division
+        ev = torch.tensor(KJMOL_PER_EV, dtype=torch.float64, device=dev)
+        self._res = dict(key=key, perm=perm, inv=inv, n=n, nsys=len(reqs), slot=slot, kjmol_per_ev=ev)
+
     def forward(self, state, **kwargs):
+        key = self._key(state)
+        res = self._res
+        if res is None or res["key"] != key or res["slot"].resident_key != key:
+            self._setup(state, key)
+            res = self._res
+        slot, n, nsys = res["slot"], res["n"], res["nsys"]
+        cur = torch.cuda.current_stream(slot.ws.device)
+        pos = torch.as_tensor(state.positions).detach().to(device=slot.ws.device, dtype=torch.float64)
+        slot.device_atoms(n)[:, 0:3] = pos.index_select(0, res["perm"]) * NM_PER_ANGSTROM
+        slot.stream.wait_stream(cur)
+        with torch.cuda.stream(slot.stream):
+            slot.run()
+        cur.wait_stream(slot.stream)
+        d = slot.d_out
+        f0 = 5 * self.engine.caps["max_systems"]
+        energy = d[4: 5 * nsys: 5] / res["kjmol_per_ev"]
+        forces = d[f0: f0 + 3 * n].view(n, 3).index_select(0, res["inv"]) * (NM_PER_ANGSTROM / KJMOL_PER_EV)
+        return {"energy": energy.to(device=self._device, dtype=self._dtype),
+                "forces": forces.to(device=self._device, dtype=self._dtype)}
+
+    def forward_via_requests(self, state):
+        """The same evaluation through the request path (host tables, one upload and one read-back
+        per call): the reference for the resident path."""
         reqs = state_requests(state, self.charges, self.sigma, self.epsilon, self.molecule, self.spacing)
         for r in reqs:
             if not self.engine.fits_alone(r):

@@ -145,6 +145,9 @@ class _Slot:
         self.event = torch.cuda.Event()
         self.graph = None
         self.bytes = nbytes + self.nout * 8
+        self.atoms_off = off
+        # the batch whose table the device workspace holds for resident use (None: any other state)
+        self.resident_key = None
 
     def launch_call(self):
         eng = self.eng
@@ -156,6 +159,20 @@ class _Slot:
         rc = L.load().tff_serve_upload(self.table_ptr, self.eng.cd, *self.eng.cap_args, L.ptr(self.ws),
                                        self.ws.numel(), self.sptr)
         L.check(rc, "serve upload")
+        self.resident_key = None
+
+    def device_atoms(self, n):
+        """(n, 8) float64 view of the first n atom records of the uploaded batch in the device
+        workspace (the workspace starts with the table's layout): writing x, y, z there between
+        evaluations moves the atoms without another upload."""
+        return self.ws[self.atoms_off: self.atoms_off + 64 * n].view(torch.float64).view(n, 8)
+
+    def run(self):
+        """The evaluation of the uploaded batch on the slot's stream (graph replay when captured)."""
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.launch_call()
 
     def reset(self):
         L.load().tff_serve_table_reset(self.table_ptr)
@@ -185,10 +202,7 @@ class _Slot:
         with torch.cuda.stream(self.stream):
             # the upload stays outside the graph: its size follows the batch
             self.upload()
-            if self.graph is not None:
-                self.graph.replay()
-            else:
-                self.launch_call()
+            self.run()
             self.h_out[:n].copy_(self.d_out[:n], non_blocking=True)
             self.event.record(self.stream)
 
