@@ -77,6 +77,8 @@ int tff_build_neighbor_list(int dtype, const void* coords, long long natoms, con
 #define TFF_PAIR_COUL_EWALD 2  /* erfc Coulomb (rank-0 multipoles) + exclusion correction */
 #define TFF_PAIR_LJ_TYPES 3    /* Lennard-Jones from a (T, T) type table */
 #define TFF_PAIR_LJ_PAIRS 4    /* Lennard-Jones from per-pair sigma / epsilon */
+#define TFF_PAIR_COUL_EWALD_SYM 5 /* COUL_EWALD with q_i q_j replaced by (a_i b_j + a_j b_i) / 2:
+                                     a = charges, b = the sigma argument; grad_coords only */
 
 size_t tff_pair_workspace_bytes(int mode, long long natoms, long long npairs, long long ntypes);
 
@@ -89,7 +91,8 @@ size_t tff_pair_workspace_bytes(int mode, long long natoms, long long npairs, lo
  * in modes LJ_TYPES / LJ_PAIRS; and the fused erfc-Coulomb + LJ extension in mode EWALD_LJ.
  *
  * Outputs (any may be NULL): energy_out (1 value) = scale * E;
- * grad_* = scale * dE/d(argument), written (not accumulated); for LJ_TYPES
+ * grad_* = scale * dE/d(argument), written (not accumulated), accumulated as exact integer
+ * sums (bitwise the same on every run of the same pair list); for LJ_TYPES
  * grad_sigma / grad_epsilon have ntypes^2 entries, for LJ_PAIRS npairs entries,
  * otherwise natoms; forces_accum += -scale * dE/dcoords.
  * pairs_excl are used only by the Ewald modes with ewald_alpha >= 0. */

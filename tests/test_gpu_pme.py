@@ -16,7 +16,7 @@ from torchff.workloads import make_workload, water_box
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "random_water_300_rank0.npz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "random_water_300.npz")
 
 
 def _ref(coords, box, q, K, alpha, order):

@@ -111,6 +111,13 @@ __device__ __forceinline__ bool tff_is_excluded(int64_t i, int64_t j, const int6
   return false;
 }
 
+// Makes this wave's LDS writes visible to its own lanes (lanes read what other lanes wrote).
+__device__ __forceinline__ void tff_wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
 __device__ __forceinline__ T tff_wave_sum(T v) {
   for (int off = TFF_WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, TFF_WAVE);

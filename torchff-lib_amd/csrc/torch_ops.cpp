@@ -4,7 +4,7 @@
 // that has only libtorch and this library loaded (no Python), e.g. behind OpenMM's TorchForce
 // (the reference's examples/tip3p/md.py:451-520).  The schemas are the reference's
 // (csrc/nblist/nblist_interface.cpp:4-9, csrc/nonbonded/nonbonded_interface.cpp:9-10,
-// csrc/multipoles/multipoles_interface.cpp:6, csrc/vdw/vdw_interface.cpp:6,
+// csrc/multipoles/multipoles_interface.cpp:6-7, csrc/vdw/vdw_interface.cpp:6,
 // csrc/pme/pme_interface.cpp:6-9), plus a trailing `int order=6` on pme_long_range and the
 // additive compute_nonbonded_ewald_energy_from_atom_pairs.  As in the reference, the
 // differentiable energies register an AutogradCUDA kernel whose forward computes the energy and
@@ -305,6 +305,87 @@ void nonbonded_forces(const Tensor& coords, const Tensor& pairs, const Tensor& b
             charges, sigma, epsilon, Tensor(), Tensor(), want, g, forces);
 }
 
+// ------------------------------------------------- rank-0 energy with per-atom potential
+// compute_multipolar_energy_and_fields_from_atom_pairs for point charges (reference
+// csrc/multipoles/multipoles_field.cu, torchff/multipoles.py:276-291): the energy E (shape (1,)),
+// the potential phi = dE/dq and the field, zero for point charges as in the reference's CUDA op.
+// E = 1/2 q^T A q with A_ij = pref T(r_ij) (erfc or bare kernel, the exclusion corrections
+// included), phi = A q.  For upstream gradients gE (of E) and w (of phi), L = gE E + w . phi:
+// dL/dq = A (gE q + w), the potential of the charges gE q + w; and
+// dL/dr = grad_r of 1/2 q^T A b with b = gE q + 2 w held fixed, the symmetric bilinear Coulomb
+// sum of q and b (TFF_PAIR_COUL_EWALD_SYM).
+struct FieldSpec {
+  double cutoff, alpha, pref;
+};
+
+Tensor coulomb_potential(const Tensor& coords, const Tensor& box, const Tensor& pairs, const Tensor& pe,
+                         const Tensor& charges, const FieldSpec& f, Tensor* energy) {
+  const bool want[4] = {false, true, false, false};
+  Tensor g[4];
+  Tensor e = pair_call(PairSpec{TFF_PAIR_COUL_EWALD, f.pref, f.cutoff, f.alpha, false, true}, coords, pairs, box,
+                       charges, Tensor(), Tensor(), Tensor(), pe, want, g, Tensor());
+  if (energy) *energy = e;
+  return g[1];
+}
+
+struct MultipoleFields : public torch::autograd::Function<MultipoleFields> {
+  static variable_list forward(AutogradContext* ctx, Tensor coords, Tensor q, Tensor box, Tensor pairs,
+                               OptT pairs_excl, double cutoff, double alpha, double pref) {
+    const FieldSpec f{cutoff, alpha, pref};
+    Tensor e;
+    const Tensor pot = coulomb_potential(coords, box, pairs, opt(pairs_excl), q, f, &e);
+    Tensor field = at::zeros({coords.size(0), 3}, coords.options());
+    ctx->mark_non_differentiable({field});
+    ctx->save_for_backward({coords.detach(), q.detach(), box, pairs, opt(pairs_excl)});
+    ctx->saved_data["cutoff"] = cutoff;
+    ctx->saved_data["alpha"] = alpha;
+    ctx->saved_data["pref"] = pref;
+    return {e, pot, field};
+  }
+  static variable_list backward(AutogradContext* ctx, variable_list grad_out) {
+    const auto sv = ctx->get_saved_variables();
+    const Tensor &c = sv[0], &q = sv[1], &box = sv[2], &pairs = sv[3], &pe = sv[4];
+    const FieldSpec f{ctx->saved_data["cutoff"].toDouble(), ctx->saved_data["alpha"].toDouble(),
+                      ctx->saved_data["pref"].toDouble()};
+    const auto dt = c.scalar_type();
+    const Tensor gE = grad_out[0].defined() ? grad_out[0].reshape({}).to(dt) : at::zeros({}, c.options());
+    const Tensor w = grad_out[1].defined() ? grad_out[1].to(dt).contiguous() : at::zeros_like(q);
+    variable_list out(8);
+    if (ctx->needs_input_grad(1)) out[1] = coulomb_potential(c, box, pairs, pe, q * gE + w, f, nullptr);
+    if (ctx->needs_input_grad(0)) {
+      const Tensor b = (q * gE + w * 2.0).contiguous();
+      const bool want[4] = {true, false, false, false};
+      Tensor g[4];
+      pair_call(PairSpec{TFF_PAIR_COUL_EWALD_SYM, f.pref, f.cutoff, f.alpha, false, true}, c, pairs, box, q, b,
+                Tensor(), Tensor(), pe, want, g, Tensor());
+      out[0] = g[0];
+    }
+    return out;
+  }
+};
+
+std::tuple<Tensor, Tensor, Tensor> multipolar_fields_ag(const Tensor& coords, const Tensor& box, const Tensor& pairs,
+                                                        const c10::optional<Tensor>& pairs_excl, const Tensor& q,
+                                                        const c10::optional<Tensor>& p, const c10::optional<Tensor>& t,
+                                                        const c10::Scalar& cutoff, const c10::Scalar& alpha,
+                                                        const c10::Scalar& prefactor) {
+  rank0_only(p, t);
+  auto r = MultipoleFields::apply(coords, q, box, pairs, some(opt(pairs_excl)), cutoff.toDouble(), alpha.toDouble(),
+                                  prefactor.toDouble());
+  return {r[0], r[1], r[2]};
+}
+std::tuple<Tensor, Tensor, Tensor> multipolar_fields_cuda(const Tensor& coords, const Tensor& box, const Tensor& pairs,
+                                                          const c10::optional<Tensor>& pairs_excl, const Tensor& q,
+                                                          const c10::optional<Tensor>& p,
+                                                          const c10::optional<Tensor>& t, const c10::Scalar& cutoff,
+                                                          const c10::Scalar& alpha, const c10::Scalar& prefactor) {
+  rank0_only(p, t);
+  Tensor e;
+  const Tensor pot = coulomb_potential(coords, box, pairs, opt(pairs_excl), q,
+                                       FieldSpec{cutoff.toDouble(), alpha.toDouble(), prefactor.toDouble()}, &e);
+  return {e, pot, at::zeros({coords.size(0), 3}, coords.options())};
+}
+
 // ---------------------------------------------------------------- PME (point charges)
 struct PmeGeometry {
   int K[3];
@@ -428,6 +509,9 @@ TORCH_LIBRARY_FRAGMENT(torchff, m) {
   m.def("compute_multipolar_energy_from_atom_pairs(Tensor coords, Tensor box, Tensor pairs, "
         "Tensor? pairs_excl, Tensor q, Tensor? p, Tensor? t, Scalar cutoff, Scalar ewald_alpha, "
         "Scalar prefactor) -> Tensor");
+  m.def("compute_multipolar_energy_and_fields_from_atom_pairs(Tensor coords, Tensor box, Tensor pairs, "
+        "Tensor? pairs_excl, Tensor q, Tensor? p, Tensor? t, Scalar cutoff, Scalar ewald_alpha, "
+        "Scalar prefactor) -> (Tensor, Tensor, Tensor)");
   m.def("compute_lennard_jones_energy(Tensor coords, Tensor pairs, Tensor box, Tensor sigma, "
         "Tensor epsilon, Scalar cutoff, Tensor? atom_types) -> Tensor");
   m.def("pme_long_range(Tensor coords, Tensor box, Tensor q, Tensor? p, Tensor? t, Scalar K, Scalar alpha, "
@@ -443,6 +527,7 @@ TORCH_LIBRARY_IMPL(torchff, CUDA, m) {
   m.impl("compute_nonbonded_energy_from_atom_pairs", &nonbonded_energy_cuda);
   m.impl("compute_nonbonded_ewald_energy_from_atom_pairs", &nonbonded_ewald_energy_cuda);
   m.impl("compute_multipolar_energy_from_atom_pairs", &multipolar_energy_cuda);
+  m.impl("compute_multipolar_energy_and_fields_from_atom_pairs", &multipolar_fields_cuda);
   m.impl("compute_lennard_jones_energy", &lj_energy_cuda);
   m.impl("pme_long_range", &pme_long_range_cuda);
 }
@@ -451,6 +536,7 @@ TORCH_LIBRARY_IMPL(torchff, AutogradCUDA, m) {
   m.impl("compute_nonbonded_energy_from_atom_pairs", &nonbonded_energy_ag);
   m.impl("compute_nonbonded_ewald_energy_from_atom_pairs", &nonbonded_ewald_energy_ag);
   m.impl("compute_multipolar_energy_from_atom_pairs", &multipolar_energy_ag);
+  m.impl("compute_multipolar_energy_and_fields_from_atom_pairs", &multipolar_fields_ag);
   m.impl("compute_lennard_jones_energy", &lj_energy_ag);
   m.impl("pme_long_range", &pme_long_range_ag);
 }

@@ -17,7 +17,7 @@ OPS_PATH = os.path.join(_HERE, "libtorchff_ops.so")
 F32, F64 = 0, 1
 NBLIST_NSQUARED, NBLIST_CELL_LIST = 0, 1
 SERVE_FULL = 5  # tff_serve_table_add: the system does not fit the capacities
-PAIR_NONBONDED, PAIR_EWALD_LJ, PAIR_COUL_EWALD, PAIR_LJ_TYPES, PAIR_LJ_PAIRS = 0, 1, 2, 3, 4
+PAIR_NONBONDED, PAIR_EWALD_LJ, PAIR_COUL_EWALD, PAIR_LJ_TYPES, PAIR_LJ_PAIRS, PAIR_COUL_EWALD_SYM = 0, 1, 2, 3, 4, 5
 
 _vp = ctypes.c_void_p
 _ll = ctypes.c_longlong

@@ -2,8 +2,11 @@
 
 Real-space Ewald Coulomb prefactor * sum q_i q_j erfc(alpha r)/r over the pair
 list (bare 1/r when ewald_alpha < 0) plus the (erfc(alpha r) - 1)/r
-correction over ``pairs_excl``.  Dipoles, quadrupoles and field outputs are
-not part of this build and raise.
+correction over ``pairs_excl``.  ``return_fields=True`` returns (energy, per-atom
+potential dE/dq, field) with the field zero for point charges, as the reference's
+CUDA op (csrc/multipoles/multipoles_field.cu), differentiable through energy and
+potential.  Dipoles and quadrupoles (ranks 1-2) in real space are not part of this
+build and raise.
 """
 
 import torch
@@ -19,8 +22,6 @@ class MultipolarInteraction(nn.Module):
         super().__init__()
         if rank != 0:
             raise NotImplementedError("MultipolarInteraction: only rank 0 is provided by the gfx950 build")
-        if return_fields:
-            raise NotImplementedError("MultipolarInteraction: return_fields is not provided by the gfx950 build")
         require_custom(use_customized_ops, "MultipolarInteraction")
         self.rank = rank
         self.cutoff = cutoff
@@ -32,5 +33,8 @@ class MultipolarInteraction(nn.Module):
 
     @torch._dynamo.disable
     def forward(self, coords, box, pairs, q, p=None, t=None, pairs_excl=None):
+        if self.return_fields:
+            return torch.ops.torchff.compute_multipolar_energy_and_fields_from_atom_pairs(
+                coords, box, pairs, pairs_excl, q, p, t, self.cutoff, self.ewald_alpha, self.prefactor)
         return torch.ops.torchff.compute_multipolar_energy_from_atom_pairs(
             coords, box, pairs, pairs_excl, q, p, t, self.cutoff, self.ewald_alpha, self.prefactor)
