@@ -243,7 +243,9 @@ int tff_cluster_pairs_nonbonded(int dtype, const void* coords, long long natoms,
                                        z, which the charge spreading's cluster search relies on
                                        (reciprocal-space results incomplete) */
 #define TFF_FUSED_BRICK_OVERFLOW 8  /* a charge-spreading brick is reached by more than 1024
-                                       clusters (strongly inhomogeneous system or a coarse grid):
+                                       clusters (float64 spread), or holds the stencil bases of
+                                       more than 4 x the mean atoms per brick + 64 (float spread):
+                                       strongly inhomogeneous system or a coarse grid;
                                        reciprocal-space results incomplete */
 #define TFF_FUSED_BOX_STALE 16      /* box_device no longer holds the values of box_host that the
                                        call was planned from: results invalid */

@@ -5,8 +5,11 @@ is detected on the device.
 Inputs that set each flag (all coordinates finite, every kernel exits normally):
   * pair capacity: the 96k-atom water box with z squeezed three-fold (the per-cluster neighbour
     count is three times what the box average sizes the segments for);
-  * brick candidates: the 96k-atom box on a 16^3 grid (one 8^3 brick covers half the box, so
-    most of the 3000 clusters reach it, above the 1024 kept per brick);
+  * brick candidates (float64 spread): the 96k-atom box on a 16^3 grid (one 8^3 brick covers half
+    the box, so most of the 3000 clusters reach it, above the 1024 kept per brick);
+  * brick list (float spread): a 6 x 6 x 6 lattice of 1/6 nm spacing inside one 1 nm brick of a
+    6 nm box on a 48^3 grid, the rest of the box a 0.43 nm lattice: 216 stencil bases in one brick
+    against a capacity of 4 x 14 + 64 = 120;
   * exclusion partners: atom 0 of the 900-atom box given nine extra partners;
   * column size: a 12 x 12 x 560 lattice (0.208 nm spacing) in a 2.5 x 2.5 x 116.7 nm box,
     8960 atoms per x-y column (above the 8192 sorted in LDS: the column stays unsorted in z).
@@ -54,10 +57,37 @@ def test_pair_overflow_raises_on_a_later_call():
 
 def test_brick_overflow_has_its_own_message():
     w = make_workload("water96k", K=16)
-    m = for_workload(w, "mixed")
+    m = for_workload(w, "fp64")
     with pytest.raises(RuntimeError, match="charge-spreading brick") as ei:
         m(*_args(w), check=True)
     assert "cluster-pair capacity" not in str(ei.value)
+
+
+def _dense_brick_box():
+    """216 atoms on a 1/6 nm lattice inside the brick [2.25, 3.25)^3 nm of a 6 nm box (grid 48^3,
+    bricks of 8 points = 1 nm), the rest of the box a 14^3 lattice of 0.43 nm spacing."""
+    L = 6.0
+    dense = (np.stack(np.meshgrid(*[np.arange(6)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) / 6.0 + 2.25
+    sparse = (np.stack(np.meshgrid(*[np.arange(14)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * (L / 14)
+    inside = np.all((sparse >= 2.0) & (sparse < 3.5), axis=1)
+    x = np.concatenate([dense, sparse[~inside]])
+    n = len(x)
+    q = np.where(np.arange(n) % 2 == 0, 0.4, -0.4)
+    return dict(coords=x, box=np.diag([L, L, L]), charges=q, sigma=np.full(n, 0.15), epsilon=np.full(n, 0.3))
+
+
+def test_brick_list_overflow_float_spread():
+    w = _dense_brick_box()
+    n = len(w["coords"])
+    assert 4 * -(-n // 216) + 64 < 216
+    m = FusedNonbondedPME(n, 0.9, ewald_alpha(0.9), (48, 48, 48), order=5, precision="mixed")
+    with pytest.raises(RuntimeError, match="charge-spreading brick") as ei:
+        m(*_args(w), check=True)
+    assert "cluster-pair capacity" not in str(ei.value)
+    # the same system on the float64 spread (no lists) is within its capacities
+    m64 = FusedNonbondedPME(n, 0.9, ewald_alpha(0.9), (48, 48, 48), order=5, precision="fp64")
+    E, F = m64(*_args(w), check=True)[:2]
+    assert torch.isfinite(E).all() and torch.isfinite(F).all()
 
 
 def test_exclusion_overflow():

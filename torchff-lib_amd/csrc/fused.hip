@@ -1176,32 +1176,14 @@ __device__ __forceinline__ void stencil_interval(float slo, float shi, int P, in
   len = hi - lo + 1;
 }
 
-// DIPOLE: instead of the charges, spread the dipoles q_i v_i . grad_r theta_i(m) (the grid of the
-// force-matching backward), from per-slot vectors vsl and the Cartesian-to-grid scale jsc = K / L.
-template <typename TC, int P, bool DIPOLE>
-__global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
-                                                       const int* __restrict__ slot_orig, const float4* __restrict__ bb_c,
-                                                       const float4* __restrict__ bb_h,
-                                                       const int* __restrict__ col_cl_start, Geo g, PmeGeom pg,
-                                                       unsigned long long* __restrict__ grid, int* __restrict__ status,
-                                                       const V4<TC>* __restrict__ vsl, V4<TC> jsc) {
-  __shared__ unsigned long long s_q[BR * BR * BR];
-  __shared__ int s_list[BRICK_MAXC];
-  __shared__ V4<TC> s_pos[SPREAD_CHUNK * CL];
-  __shared__ int s_slot[DIPOLE ? SPREAD_CHUNK * CL : 1];
-  __shared__ int s_coff[2 * SPREAD_MAXCOL + 1], s_ccs[2 * SPREAD_MAXCOL], s_cln[2 * SPREAD_MAXCOL];
-  __shared__ int s_n, s_na;
-  const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
-  const int nbx = (K0 + BR - 1) / BR, nby = (K1 + BR - 1) / BR, nbz = (K2 + BR - 1) / BR;
-  const int bid = blockIdx.x;
-  if (bid >= nbx * nby * nbz) return;
-  const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
-  const int B0[3] = {bx * BR, by * BR, bz * BR};
-  const int Bn[3] = {K0 - B0[0] < BR ? K0 - B0[0] : BR, K1 - B0[1] < BR ? K1 - B0[1] : BR,
-                     K2 - B0[2] < BR ? K2 - B0[2] : BR};
-  for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) s_q[k] = 0ull;
-  if (threadIdx.x == 0) s_n = 0;
-  __syncthreads();
+// Lists in s_list (at most BRICK_MAXC entries; *s_n gets the full count) the clusters whose
+// stencils can reach the brick at B0 of extent Bn.  Called by every thread of the workgroup; *s_n
+// must be zero (and visible) on entry.
+template <int P>
+__device__ __forceinline__ void brick_clusters(const float4* __restrict__ bb_c, const float4* __restrict__ bb_h,
+                                               const int* __restrict__ col_cl_start, const Geo& g,
+                                               const PmeGeom& pg, const int (&B0)[3], const int (&Bn)[3],
+                                               int* s_list, int* s_coff, int* s_ccs, int* s_cln, int* s_n) {
   // candidate columns: those whose x-y extent can reach the brick
   const float kl[3] = {(float)(pg.K[0] * g.invL[0]), (float)(pg.K[1] * g.invL[1]), (float)(pg.K[2] * g.invL[2])};
   int c0[2], nc[2];
@@ -1304,12 +1286,41 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
         ok = ok && circ_overlap(lo2, len, B0[a], Bn[a], pg.K[a]);
       }
       if (ok) {
-        const int q = atomicAdd(&s_n, 1);
+        const int q = atomicAdd(s_n, 1);
         if (q < BRICK_MAXC) s_list[q] = c;
       }
     }
     __syncthreads();
   }
+}
+
+// DIPOLE: instead of the charges, spread the dipoles q_i v_i . grad_r theta_i(m) (the grid of the
+// force-matching backward), from per-slot vectors vsl and the Cartesian-to-grid scale jsc = K / L.
+template <typename TC, int P, bool DIPOLE>
+__global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ ncl_ptr, const V4<TC>* __restrict__ xyzq,
+                                                       const int* __restrict__ slot_orig, const float4* __restrict__ bb_c,
+                                                       const float4* __restrict__ bb_h,
+                                                       const int* __restrict__ col_cl_start, Geo g, PmeGeom pg,
+                                                       unsigned long long* __restrict__ grid, int* __restrict__ status,
+                                                       const V4<TC>* __restrict__ vsl, V4<TC> jsc) {
+  __shared__ unsigned long long s_q[BR * BR * BR];
+  __shared__ int s_list[BRICK_MAXC];
+  __shared__ V4<TC> s_pos[SPREAD_CHUNK * CL];
+  __shared__ int s_slot[DIPOLE ? SPREAD_CHUNK * CL : 1];
+  __shared__ int s_coff[2 * SPREAD_MAXCOL + 1], s_ccs[2 * SPREAD_MAXCOL], s_cln[2 * SPREAD_MAXCOL];
+  __shared__ int s_n, s_na;
+  const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
+  const int nbx = (K0 + BR - 1) / BR, nby = (K1 + BR - 1) / BR, nbz = (K2 + BR - 1) / BR;
+  const int bid = blockIdx.x;
+  if (bid >= nbx * nby * nbz) return;
+  const int bz = bid % nbz, by = (bid / nbz) % nby, bx = bid / (nbz * nby);
+  const int B0[3] = {bx * BR, by * BR, bz * BR};
+  const int Bn[3] = {K0 - B0[0] < BR ? K0 - B0[0] : BR, K1 - B0[1] < BR ? K1 - B0[1] : BR,
+                     K2 - B0[2] < BR ? K2 - B0[2] : BR};
+  for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) s_q[k] = 0ull;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  brick_clusters<P>(bb_c, bb_h, col_cl_start, g, pg, B0, Bn, s_list, s_coff, s_ccs, s_cln, &s_n);
   int ncand = s_n;
   if (ncand > BRICK_MAXC) {
     if (threadIdx.x == 0) atomicOr(status, TFF_FUSED_BRICK_OVERFLOW);
@@ -1407,6 +1418,275 @@ __global__ __launch_bounds__(256) void k_spread_bricks(const int* __restrict__ n
   }
   __syncthreads();
   for (int k = threadIdx.x; k < BR * BR * BR; k += blockDim.x) {
+    const int lz = k % BR, ly = (k / BR) % BR, lx = k / (BR * BR);
+    if (lx < Bn[0] && ly < Bn[1] && lz < Bn[2])
+      grid[((long long)(B0[0] + lx) * K1 + (B0[1] + ly)) * K2 + (B0[2] + lz)] = s_q[k];
+  }
+}
+
+// ---- charge spread of the float path: per-slot spline tables + home-brick lists
+// A slot's home brick is the brick holding its stencil base (the grid point of its first spline
+// weight on every axis).  k_slot_splines evaluates each slot's spline weights once (the
+// arithmetic of k_spread_bricks and k_gather_slots) into wts[sl * WST ...] = M(t_x + k),
+// M(t_y + k), M(t_z + k) for k < P, then q, and appends (slot, packed base) to its home brick's
+// list (capacity cap per brick; the lanes of a wave that share a home brick reserve one range
+// with one atomic).  k_spread_lists then spreads each brick from the lists of the home bricks whose
+// stencils can reach it: no search over clusters.  Needs P <= 6 and BR + P <= K_a <= 1023 (every
+// stencil meets a brick in one run of grid points; bases fit 10 bits).
+template <int P> constexpr int spline_stride() { return (3 * P + 1 + 3) & ~3; }
+constexpr int LIST_WST = 20;  // spline_stride<6>(): the table stride of the largest order served
+
+struct BrickGrid {
+  int nb[3];  // bricks per axis
+  int cap;    // list entries per brick
+};
+
+template <int P>
+__global__ __launch_bounds__(FB) void k_slot_splines(const int* __restrict__ ncl_ptr, const V4<float>* __restrict__ xyzq,
+                                                     const int* __restrict__ slot_orig, PmeGeom pg, BrickGrid bg,
+                                                     float* __restrict__ wts, int* __restrict__ bcount,
+                                                     int2* __restrict__ blist, int* __restrict__ status) {
+  const int sl = blockIdx.x * FB + threadIdx.x;
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  const bool valid = sl < *ncl_ptr * CL && slot_orig[sl] >= 0;
+  int hb = -1, packed = 0;
+  if (valid) {
+    const V4<float> p = xyzq[sl];
+    const double rr[3] = {(double)p.x, (double)p.y, (double)p.z};
+    int base[3];
+    float* w = wts + (size_t)sl * LIST_WST;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double sa = (rr[0] * pg.inv[a] + rr[1] * pg.inv[3 + a] + rr[2] * pg.inv[6 + a]) * (double)pg.K[a];
+      double t;
+      float Mv[P], dM[P];
+      anchor_axis(sa, pg.K[a], P, base[a], t);
+      spline_weights<float, P>((float)t, Mv, dM);
+#pragma unroll
+      for (int k = 0; k < P; ++k) w[a * P + k] = Mv[k];
+    }
+    w[3 * P] = p.w;
+    hb = ((base[0] / BR) * bg.nb[1] + base[1] / BR) * bg.nb[2] + base[2] / BR;
+    packed = base[0] | (base[1] << 10) | (base[2] << 20);
+  }
+  // wave-aggregated append: group the lanes by home brick (ballots only), then every group's first
+  // lane reserves the group's range, all groups at once
+  unsigned long long todo = __ballot(valid), grp = 0;
+  int leader = 0;
+  while (todo) {
+    const int l = __ffsll((long long)todo) - 1;
+    const int h = __shfl(hb, l, TFF_WAVE);
+    const unsigned long long m = __ballot(valid && hb == h);
+    todo &= ~m;
+    if (valid && hb == h) {
+      grp = m;
+      leader = l;
+    }
+  }
+  int start = 0;
+  if (valid && lane == leader) start = atomicAdd(&bcount[hb], __popcll(grp));
+  start = __shfl(start, leader, TFF_WAVE);
+  if (valid) {
+    const int r = start + __popcll(grp & ((1ull << lane) - 1ull));
+    if (r < bg.cap)
+      blist[(long long)hb * bg.cap + r] = make_int2(sl, packed);
+    else
+      atomicOr(status, TFF_FUSED_BRICK_OVERFLOW);
+  }
+}
+
+// Charge spread from the home-brick lists: one workgroup per BR^3 brick, in rounds of up to 256
+// listed slots.  Each thread tests one slot's stencil against the brick (one run of grid points per
+// axis) and compacts into LDS the slot's weights over its in-brick runs (q M_x, M_y, M_z, zero past
+// the run) and the runs' starts and lengths.  Then lane k takes x point k % P of compacted slot
+// k / P and adds its y-z box of points ((q M_x M_y) M_z, the products k_spread_bricks forms) to the
+// brick in LDS with int64 fixed-point atomics.  Integer sums do not depend on the order, so the grid
+// equals k_spread_bricks' bit for bit; the brick is stored once, so the grid needs neither clearing
+// nor global atomics.
+constexpr int LIST_ROUND = 256;
+template <int P>
+__global__ __launch_bounds__(LIST_ROUND) void k_spread_lists(const float* __restrict__ wts,
+                                                             const int* __restrict__ bcount,
+                                                             const int2* __restrict__ blist, BrickGrid bg,
+                                                             PmeGeom pg, unsigned long long* __restrict__ grid) {
+  constexpr int NT = LIST_ROUND;
+  constexpr int REC = spline_stride<P>();  // per compacted slot: the weights of its runs, then the runs
+  __shared__ unsigned long long s_q[BR * BR * BR];
+  __shared__ __align__(16) float s_w[NT][REC];
+  __shared__ int s_hoff[28], s_hid[27], s_cnt[27];
+  __shared__ int s_na, s_nh;
+  const int K0 = pg.K[0], K1 = pg.K[1], K2 = pg.K[2];
+  const int bid = blockIdx.x;
+  if (bid >= bg.nb[0] * bg.nb[1] * bg.nb[2]) return;
+  const int bi[3] = {bid / (bg.nb[2] * bg.nb[1]), (bid / bg.nb[2]) % bg.nb[1], bid % bg.nb[2]};
+  const int Ks[3] = {K0, K1, K2};
+  int B0[3], Bn[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    B0[a] = bi[a] * BR;
+    Bn[a] = Ks[a] - B0[a] < BR ? Ks[a] - B0[a] : BR;
+  }
+  for (int k = threadIdx.x; k < BR * BR * BR; k += NT) s_q[k] = 0ull;
+  // home bricks whose slots' stencils can reach this brick: per axis the distinct h among b, b - 1,
+  // b - 2 (mod the brick count) whose base range plus stencil meets the brick; lane t < 27 of the
+  // first wave takes combination t and loads its list length (the other waves skip this scalar
+  // work: the CU's scalar unit is shared)
+  const int t = threadIdx.x;
+  if (t < TFF_WAVE) {
+    int hl[3][3], hn[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) {
+      if (Ks[a] % BR == 0 && bg.nb[a] >= 2) {
+        // full bricks: the brick itself and the one below (P <= BR + 1)
+        hl[a][0] = bi[a];
+        hl[a][1] = bi[a] > 0 ? bi[a] - 1 : bg.nb[a] - 1;
+        hn[a] = 2;
+        continue;
+      }
+      for (int d = 0; d < 3; ++d) {
+        int h = bi[a] - d;
+        while (h < 0) h += bg.nb[a];
+        bool dup = false;
+        for (int e = 0; e < hn[a]; ++e) dup = dup || hl[a][e] == h;
+        const int hB = Ks[a] - h * BR < BR ? Ks[a] - h * BR : BR;
+        if (!dup && circ_overlap(h * BR, hB + P - 1, B0[a], Bn[a], Ks[a])) hl[a][hn[a]++] = h;
+      }
+    }
+    const int nh = hn[0] * hn[1] * hn[2];
+    if (t < nh) {
+      const int x = t / (hn[1] * hn[2]), y = (t / hn[2]) % hn[1], z = t % hn[2];
+      const int h = (hl[0][x] * bg.nb[1] + hl[1][y]) * bg.nb[2] + hl[2][z];
+      const int c = bcount[h];
+      s_hid[t] = h;
+      s_cnt[t] = c < bg.cap ? c : bg.cap;
+    }
+    if (t == 0) s_nh = nh;
+  }
+  {
+    __syncthreads();
+    const int nh = s_nh;
+    if (t == 0) {
+      int run = 0;
+      for (int k = 0; k < nh; ++k) {
+        s_hoff[k] = run;
+        run += s_cnt[k];
+      }
+      s_hoff[nh] = run;
+    }
+    __syncthreads();
+  }
+  const int nh = s_nh, total = s_hoff[nh];
+  // candidates in passes of 2 NT: each thread loads and tests its two (k and k + NT) and loads the
+  // spline tables of those that hit before the first is spread, so the global loads of both rounds
+  // overlap
+  for (int p0 = 0; p0 < total; p0 += 2 * NT) {
+    float rec2[2][REC];
+    bool hit2[2] = {false, false};
+    int2 e2[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int k = p0 + c * NT + threadIdx.x;
+      e2[c] = make_int2(0, 0);
+      if (k < total) {
+        int hi = 0;
+        for (int j = 1; j < nh; ++j) hi += s_hoff[j] <= k ? 1 : 0;
+        e2[c] = blist[(long long)s_hid[hi] * bg.cap + (k - s_hoff[hi])];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int k = p0 + c * NT + threadIdx.x;
+      if (k >= total) continue;
+      const int2 e = e2[c];
+      const int bs[3] = {e.y & 1023, (e.y >> 10) & 1023, (e.y >> 20) & 1023};
+      bool hit = true;
+      int q0[3], l0[3], nr[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        // the stencil [bs, bs + P) and the brick [B0, B0 + Bn) as arcs of Z_K: one common run
+        int d = B0[a] - bs[a];
+        d = d < 0 ? d + Ks[a] : d;  // brick start relative to the stencil start
+        if (d < P) {
+          q0[a] = d;
+          l0[a] = 0;
+          nr[a] = P - d < Bn[a] ? P - d : Bn[a];
+        } else {
+          int f = bs[a] - B0[a];
+          f = f < 0 ? f + Ks[a] : f;  // stencil start relative to the brick start
+          q0[a] = 0;
+          l0[a] = f;
+          nr[a] = f < Bn[a] ? (P < Bn[a] - f ? P : Bn[a] - f) : 0;
+        }
+        hit = hit && nr[a] > 0;
+      }
+      if (!hit) continue;
+      hit2[c] = true;
+      const float* src = wts + (size_t)e.x * LIST_WST;
+      float tb[3 * P + 1];
+#pragma unroll
+      for (int i = 0; i <= 3 * P; ++i) tb[i] = src[i];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+          float v = 0.f;
+#pragma unroll
+          for (int kk = 0; kk < P; ++kk)
+            if (i < nr[a] && q0[a] + i == kk) v = tb[a * P + kk];
+          // z weights carry the fixed-point scale (a power of two: (a b) 2^28 = a (b 2^28) exactly)
+          rec2[c][a * P + i] = a == 0 ? tb[3 * P] * v : (a == 2 ? v * (float)QF_SCALE : v);
+        }
+      rec2[c][3 * P] = __int_as_float(l0[0] | (l0[1] << 3) | (l0[2] << 6) | (nr[0] << 9) | (nr[1] << 13) |
+                                      (nr[2] << 17));
+#pragma unroll
+      for (int i = 3 * P + 1; i < REC; ++i) rec2[c][i] = 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (p0 + c * NT >= total) break;
+      if (threadIdx.x == 0) s_na = 0;
+      __syncthreads();
+      // 1. compact
+      if (hit2[c]) {
+        const int idx = atomicAdd(&s_na, 1);
+        float4* dst = reinterpret_cast<float4*>(s_w[idx]);
+#pragma unroll
+        for (int i = 0; i < REC / 4; ++i)
+          dst[i] = make_float4(rec2[c][4 * i], rec2[c][4 * i + 1], rec2[c][4 * i + 2], rec2[c][4 * i + 3]);
+      }
+      __syncthreads();
+      // 2. lane k: x point k % P of compacted slot k / P, its y-z box
+      const int nlanes = s_na * P;
+      for (int k = threadIdx.x; k < nlanes; k += NT) {
+        const int t = k / P, ix = k - t * P;
+        const float* w = s_w[t];
+        const int meta = __float_as_int(w[3 * P]);
+        const int nx = (meta >> 9) & 15, ny = (meta >> 13) & 15, nz = (meta >> 17) & 15;
+        if (ix >= nx) continue;
+        const float wx = w[ix];
+        float wy[P], wz[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+          wy[i] = w[P + i];
+          wz[i] = w[2 * P + i];
+        }
+        const int lx = (meta & 7) + ix, ly0 = (meta >> 3) & 7, lz0 = (meta >> 6) & 7;
+#pragma unroll
+        for (int iy = 0; iy < P; ++iy) {
+          if (iy >= ny) break;
+          const float wab = wx * wy[iy];
+          long long v[P];
+#pragma unroll
+          for (int iz = 0; iz < P; ++iz) v[iz] = (long long)__float2int_rn(wab * wz[iz]);
+          unsigned long long* row = s_q + (lx * BR + ly0 + iy) * BR + lz0;
+#pragma unroll
+          for (int iz = 0; iz < P; ++iz)
+            if (iz < nz) atomicAdd(&row[iz], (unsigned long long)v[iz]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < BR * BR * BR; k += NT) {
     const int lz = k % BR, ly = (k / BR) % BR, lx = k / (BR * BR);
     if (lx < Bn[0] && ly < Bn[1] && lz < Bn[2])
       grid[((long long)(B0[0] + lx) * K1 + (B0[1] + ly)) * K2 + (B0[2] + lz)] = s_q[k];
@@ -1699,6 +1979,11 @@ struct FusedWs {
   void* work;
   double *kparts, *qparts;
   unsigned long long* eacc;
+  // float compute only: per-slot spline tables and home-brick lists of the charge spread
+  float* wts;
+  int* bcount;
+  int2* blist;
+  BrickGrid bg;
   // backward only
   void* vsl;
   unsigned long long *gacc, *gpme, *grid_d;
@@ -1752,6 +2037,15 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc, 
   w.kparts = (double*)take(8 * (size_t)w.nk);
   w.nq = (int)((nn + FB - 1) / FB);
   w.qparts = (double*)take(8 * (size_t)w.nq);
+  if (tc == 4) {
+    for (int a = 0; a < 3; ++a) w.bg.nb[a] = (K[a] + BR - 1) / BR;
+    const long long nbr = (long long)w.bg.nb[0] * w.bg.nb[1] * w.bg.nb[2];
+    // four times the mean occupancy (plus a margin) per brick; more sets TFF_FUSED_BRICK_OVERFLOW
+    w.bg.cap = (int)(4 * ((long long)nn + nbr - 1) / nbr + 64);
+    w.wts = (float*)take(4 * (size_t)LIST_WST * slots);
+    w.bcount = (int*)take(4 * (size_t)nbr);
+    w.blist = (int2*)take(8 * (size_t)nbr * w.bg.cap);
+  }
   if (grad) {
     w.vsl = take(4 * tc * slots);
     w.gacc = (unsigned long long*)take(8 * GN * slots);
@@ -1762,13 +2056,27 @@ FusedWs carve(void* base, long long n, const Geo& g, const int K[3], size_t tc, 
   return w;
 }
 
+// the list spread serves the float path for orders up to 6 on grids with BR + P <= K_a <= 1023
+bool spread_from_lists(int P, const int K[3]) {
+  if (P > 6) return false;
+  for (int a = 0; a < 3; ++a)
+    if (K[a] < BR + P || K[a] > 1023) return false;
+  return true;
+}
+
 template <typename TC>
 int launch_pme_slots(int order, const FusedWs& w, const Geo& g, const PmeGeom& pg, int nslots, const TC* phi,
                      double coul, bool spread, int* status, hipStream_t st) {
   unsigned nb = (unsigned)((nslots + FB - 1) / FB);
   const unsigned nbricks = (unsigned)(((pg.K[0] + BR - 1) / BR) * ((pg.K[1] + BR - 1) / BR) * ((pg.K[2] + BR - 1) / BR));
 #define TFF_SP(P)                                                                                               \
-  if (spread)                                                                                                   \
+  if (spread && sizeof(TC) == 4 && spread_from_lists(P, pg.K)) {                                                \
+    hipLaunchKernelGGL((k_slot_splines<P < 7 ? P : 6>), dim3(nb), dim3(FB), 0, st, w.ncl,                      \
+                       (const V4<float>*)w.xyzq, w.slot_orig, pg, w.bg, w.wts, w.bcount, w.blist, status);      \
+    hipLaunchKernelGGL((k_spread_lists<P < 7 ? P : 6>), dim3(nbricks), dim3(LIST_ROUND), 0, st,              \
+                       (const float*)w.wts,                                                                      \
+                       (const int*)w.bcount, (const int2*)w.blist, w.bg, pg, w.grid);                         \
+  } else if (spread)                                                                                            \
     hipLaunchKernelGGL((k_spread_bricks<TC, P, false>), dim3(nbricks), dim3(256), 0, st, w.ncl,               \
                        (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status, \
                        (const V4<TC>*)nullptr, V4<TC>{});                                                        \
@@ -1922,6 +2230,7 @@ int run_fused(const TI* coords, const TI* q, const TI* sig, const TI* eps, long 
     add(w.facc, 6LL * nslots);
     add(w.slot_orig, nslots);
     add(w.eacc, 4 * E_SLOTS);
+    if (w.bcount) add(w.bcount, (long long)w.bg.nb[0] * w.bg.nb[1] * w.bg.nb[2]);
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, cl);
   }
   // 1. sort

@@ -41,8 +41,9 @@ INVALID = PAIR_OVERFLOW | EXCL_OVERFLOW | COLUMN_OVERFLOW | BRICK_OVERFLOW | BOX
 _MESSAGES = (
     (PAIR_OVERFLOW, "cluster-pair capacity exceeded (neighbour count per cluster far above the box average)"),
     (EXCL_OVERFLOW, "an atom has more than 8 exclusion partners"),
-    (BRICK_OVERFLOW, "a charge-spreading brick is reached by more than 1024 clusters (strongly inhomogeneous "
-                     "system or a grid much coarser than the atom spacing)"),
+    (BRICK_OVERFLOW, "a charge-spreading brick is reached by more than 1024 clusters, or holds the stencil "
+                     "bases of more than four times the mean number of atoms per brick plus 64 (strongly "
+                     "inhomogeneous system or a grid much coarser than the atom spacing)"),
     (BOX_STALE, "the box tensor was changed without a version bump (through .data, a view or a graph "
                 "replay) after its host copy was cached; call torchff.invalidate_box_cache(box) or "
                 "torchff.set_box_cache(False)"),
