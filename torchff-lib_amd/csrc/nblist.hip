@@ -14,12 +14,18 @@
 //
 // Cell search: atoms are binned in fractional space into nc^3 cells of
 // width >= cutoff/2 along every reciprocal direction (or >= cutoff when the
-// box is too small for 5 cells), counting-sorted by cell, and one wavefront
-// per home cell walks the half shell of 62 (or 13) neighbour cells plus the
-// home cell.  The ni x nj candidate block of a cell pair is spread over the 64
-// lanes, so lane utilisation does not depend on cell occupancy.  Accepted
-// pairs are compacted with a wave ballot into a per-wave LDS buffer that is
-// flushed with one global atomic per 448 pairs.
+// box is too small for 5 cells), counting-sorted by cell, and one workgroup
+// of four waves per home cell walks the half shell of 62 (or 13) neighbour
+// cells plus the home cell, the home atoms and their exclusion partners staged
+// in LDS once and each neighbour cell's atoms staged by the wave that takes
+// it.  The ni x nj candidate block of a cell pair is spread over the 64 lanes,
+// so lane utilisation does not depend on cell occupancy.  Two passes: the
+// first counts each wave's accepted pairs, a scan turns the counts into
+// offsets, the second repeats the walk and writes each wave's pairs from its
+// offset in ballot order (no shared counter: one global atomic per few
+// hundred pairs on a single address cost more than the search itself), so the
+// output order is the same on every call.  The rows past the found count are
+// set to -1 after the search (each output row is written once).
 
 #include <stdarg.h>
 #include <stdio.h>
@@ -77,6 +83,14 @@ __device__ __forceinline__ void push(PairBuf& pb, bool ok, long long a, long lon
   }
   pb.n += cnt;
   if (pb.n > NB_BUF - TFF_WAVE) flush(pb, lane, pairs, max_npairs, npairs);
+}
+
+// rows from the found count on (the found rows are already written) -> -1
+__global__ void k_fill_tail(long long* pairs, long long max_npairs, const int* npairs) {
+  const long long n0 = (long long)*npairs < max_npairs ? (long long)*npairs : max_npairs;
+  long long i = 2 * n0 + blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < 2 * max_npairs; i += stride) pairs[i] = -1;
 }
 
 __global__ void k_fill_pairs(long long* pairs, long long n2, int* npairs) {
@@ -144,6 +158,12 @@ struct CellGrid {
   int nc[3];
   int reach;  // 1 or 2 cells
   int ncell;
+  // float32 prefilter of the cutoff test (orthorhombic box, every axis at least 2 reach + 3 cells,
+  // so a cell pair's periodic shift gives the minimum image): candidates whose float32 squared
+  // distance is below lo are in range, above hi out of range, the rest take the float64 predicate
+  int pf;
+  float lo, hi;
+  float L[3];
 };
 
 __global__ void k_bin(const double4* __restrict__ xyz, long long n, TffBox box, CellGrid g,
@@ -190,80 +210,218 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ count, in
   if (t == 1023) start[m] = s_tot[1023];
 }
 
-__global__ void k_scatter(const double4* __restrict__ xyz, long long n, const int* __restrict__ cell_of,
+// also the prefilter's float32 copy (orthorhombic boxes): coordinates wrapped into the box, the atom
+// index in w (bit pattern)
+__global__ void k_scatter(const double4* __restrict__ xyz, long long n, TffBox box, const int* __restrict__ cell_of,
                           const int* __restrict__ start, int* __restrict__ cursor,
-                          int* __restrict__ sorted_idx, double4* __restrict__ sorted_xyz) {
+                          int* __restrict__ sorted_idx, double4* __restrict__ sorted_xyz,
+                          float4* __restrict__ sorted_f) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   int c = cell_of[i];
   int pos = start[c] + atomicAdd(&cursor[c], 1);
   sorted_idx[pos] = (int)i;
-  sorted_xyz[pos] = xyz[i];
+  const double4 r = xyz[i];
+  sorted_xyz[pos] = r;
+  if (box.ortho) {
+    const double f[3] = {r.x * box.inv[0], r.y * box.inv[4], r.z * box.inv[8]};
+    sorted_f[pos] = make_float4((float)(r.x - box.b[0] * floor(f[0])), (float)(r.y - box.b[4] * floor(f[1])),
+                                (float)(r.z - box.b[8] * floor(f[2])), __int_as_float((int)i));
+  }
 }
 
-__global__ __launch_bounds__(NB_BLOCK) void k_cell_pairs(const int* __restrict__ start,
-                                                         const int* __restrict__ sorted_idx,
-                                                         const double4* __restrict__ sxyz, CellGrid g,
-                                                         TffBox box, double cutoff2, int include_self,
-                                                         const int64_t* __restrict__ row_ptr,
-                                                         const int64_t* __restrict__ col,
-                                                         long long* __restrict__ pairs, long long max_npairs,
-                                                         int* __restrict__ npairs) {
-  __shared__ long long sbuf[NB_WAVES][2 * NB_BUF];
+// One workgroup of CP_WAVES waves per home cell.  The home cell's atoms (in chunks of 64) and their
+// exclusion partners (up to CP_EX each; more are looked up in the CSR arrays) are staged in LDS
+// once; each wave takes every CP_WAVES-th cell of the half shell (the cells' starts and sizes loaded
+// at once, one lane per cell), stages that cell's atoms (first 64 prefetched into registers while
+// the previous cell is walked) in its own LDS area and walks the ni x nj candidate block 64 at a
+// time, the lane's (a, b) advanced incrementally (one division per cell).  WRITE = false: each
+// wave's accepted-pair count into wcount[home * CP_WAVES + wave]; WRITE = true: the pairs from
+// woff[home * CP_WAVES + wave] on, in ballot order (rows at or past max_npairs are not stored).
+constexpr int CP_WAVES = 4;
+constexpr int CP_HOME = 64;  // home atoms per chunk
+constexpr int CP_EX = 4;     // exclusion partners per home atom kept in LDS
+
+template <bool WRITE>
+__global__ __launch_bounds__(CP_WAVES * TFF_WAVE) void k_cell_pairs(
+    const int* __restrict__ start, const int* __restrict__ sorted_idx, const double4* __restrict__ sxyz,
+    const float4* __restrict__ sf, CellGrid g, TffBox box, double cutoff2, int include_self,
+    const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col, int* __restrict__ wcount,
+    const int* __restrict__ woff, long long* __restrict__ pairs, long long max_npairs, int* __restrict__ npairs) {
+  // staged atoms: x, y, z and the atom index (exact in a double) in w
+  __shared__ double4 s_h[CP_HOME];
+  // the home atoms' exclusion partners, -1 padded; all -2: more than CP_EX (CSR lookup)
+  __shared__ int4 s_hx[CP_HOME];
+  __shared__ double4 s_n[CP_WAVES][TFF_WAVE];
+  // prefilter copies (wrapped float32 coordinates, atom index bits in w); the neighbour copies carry
+  // their cell pair's periodic shift
+  __shared__ float4 s_hf[CP_HOME];
+  __shared__ float4 s_nf[CP_WAVES][TFF_WAVE];
   const int lane = threadIdx.x & (TFF_WAVE - 1);
-  const int w = threadIdx.x / TFF_WAVE;
-  PairBuf pb{sbuf[w], 0};
-  const int home = blockIdx.x * NB_WAVES + w;
-  if (home < g.ncell) {
-    const int hs = start[home];
-    const int ni = start[home + 1] - hs;
-    if (ni > 0) {
-      const int hz = home % g.nc[2];
-      const int hy = (home / g.nc[2]) % g.nc[1];
-      const int hx = home / (g.nc[2] * g.nc[1]);
-      const int R = g.reach;
-      const int side = 2 * R + 1;
-      const int noff = side * side * side;
-      // half shell: offsets with linear index >= centre (centre first is the home cell)
-      for (int o = noff / 2; o < noff; ++o) {
-        int ox = o / (side * side) - R, oy = (o / side) % side - R, oz = o % side - R;
-        int nx = hx + ox, ny = hy + oy, nz = hz + oz;
-        nx += nx < 0 ? g.nc[0] : (nx >= g.nc[0] ? -g.nc[0] : 0);
-        ny += ny < 0 ? g.nc[1] : (ny >= g.nc[1] ? -g.nc[1] : 0);
-        nz += nz < 0 ? g.nc[2] : (nz >= g.nc[2] ? -g.nc[2] : 0);
-        const int nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
-        const int js = start[nb];
-        const int nj = start[nb + 1] - js;
-        const bool self_cell = (o == noff / 2);
-        const int total = ni * nj;
-        for (int t0 = 0; t0 < total; t0 += TFF_WAVE) {
-          int t = t0 + lane;
-          bool ok = false;
-          long long ai = 0, bj = 0;
-          if (t < total) {
-            int a = t / nj, b = t - (t / nj) * nj;
-            if (!self_cell || a <= b) {
-              int pa = hs + a, pbi = js + b;
-              ai = sorted_idx[pa];
-              bj = sorted_idx[pbi];
-              if (self_cell && a == b) {
-                ok = include_self != 0;
-              } else {
-                double4 ra = sxyz[pa], rb = sxyz[pbi];
-                double d2 = tff_dist2_d(box, ra.x - rb.x, ra.y - rb.y, ra.z - rb.z);
-                ok = d2 <= cutoff2;
-              }
-              if (ok) ok = !tff_is_excluded(ai, bj, row_ptr, col);
+  const int wv = threadIdx.x / TFF_WAVE;
+  const int home = blockIdx.x;
+  if (home >= g.ncell) return;
+  // WRITE: the next row of this wave; else its count
+  long long pos = WRITE ? (long long)woff[home * CP_WAVES + wv] : 0;
+  if (WRITE && home == 0 && threadIdx.x == 0) *npairs = woff[g.ncell * CP_WAVES];
+  const int hs = start[home];
+  const int ni_all = start[home + 1] - hs;
+  const int hz = home % g.nc[2];
+  const int hy = (home / g.nc[2]) % g.nc[1];
+  const int hx = home / (g.nc[2] * g.nc[1]);
+  const int R = g.reach;
+  const int side = 2 * R + 1;
+  const int noff = side * side * side;
+  for (int h0 = 0; h0 < ni_all; h0 += CP_HOME) {
+    const int ni = ni_all - h0 < CP_HOME ? ni_all - h0 : CP_HOME;
+    __syncthreads();  // the previous chunk's readers are done
+    if (threadIdx.x < ni) {
+      const int pa = hs + h0 + threadIdx.x;
+      const int ai = sorted_idx[pa];
+      double4 r = sxyz[pa];
+      r.w = (double)ai;
+      s_h[threadIdx.x] = r;
+      if (g.pf) s_hf[threadIdx.x] = sf[pa];
+      int ex[CP_EX] = {-1, -1, -1, -1};
+      if (col) {
+        const int64_t e0 = row_ptr[ai], e1 = row_ptr[ai + 1];
+        if (e1 - e0 <= CP_EX) {
+          for (int k = 0; k < CP_EX; ++k)
+            if (k < e1 - e0) ex[k] = (int)col[e0 + k];
+        } else {
+          for (int k = 0; k < CP_EX; ++k) ex[k] = -2;
+        }
+      }
+      s_hx[threadIdx.x] = make_int4(ex[0], ex[1], ex[2], ex[3]);
+    }
+    __syncthreads();
+    // this wave's cells of the half shell (offsets with linear index >= centre, the centre first
+    // being the home cell): o = noff / 2 + wv + CP_WAVES c; lane c holds cell c's start and size
+    const int ncw = (noff - noff / 2 - wv + CP_WAVES - 1) / CP_WAVES;
+    int my_js = 0, my_nj = 0;
+    float my_sx = 0.f, my_sy = 0.f, my_sz = 0.f;  // the cell's periodic shift (prefilter)
+    if (lane < ncw) {
+      const int o = noff / 2 + wv + CP_WAVES * lane;
+      const int ox = o / (side * side) - R, oy = (o / side) % side - R, oz = o % side - R;
+      int nx = hx + ox, ny = hy + oy, nz = hz + oz;
+      my_sx = nx < 0 ? -g.L[0] : (nx >= g.nc[0] ? g.L[0] : 0.f);
+      my_sy = ny < 0 ? -g.L[1] : (ny >= g.nc[1] ? g.L[1] : 0.f);
+      my_sz = nz < 0 ? -g.L[2] : (nz >= g.nc[2] ? g.L[2] : 0.f);
+      nx += nx < 0 ? g.nc[0] : (nx >= g.nc[0] ? -g.nc[0] : 0);
+      ny += ny < 0 ? g.nc[1] : (ny >= g.nc[1] ? -g.nc[1] : 0);
+      nz += nz < 0 ? g.nc[2] : (nz >= g.nc[2] ? -g.nc[2] : 0);
+      const int nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
+      my_js = start[nb];
+      my_nj = start[nb + 1] - my_js;
+    }
+    // the first 64 atoms of the next cell are loaded into registers while the current one is walked
+    double4 pre_x = make_double4(0, 0, 0, 0);
+    float4 pre_f = make_float4(0, 0, 0, 0);
+    int pre_i = 0;
+    if (ncw > 0) {
+      const int js = __shfl(my_js, 0, TFF_WAVE), nj = __shfl(my_nj, 0, TFF_WAVE);
+      if (lane < nj) {
+        pre_x = sxyz[js + lane];
+        pre_i = sorted_idx[js + lane];
+        if (g.pf) pre_f = sf[js + lane];
+      }
+    }
+    for (int c = 0; c < ncw; ++c) {
+      const int js = __shfl(my_js, c, TFF_WAVE);
+      const int nj_all = __shfl(my_nj, c, TFF_WAVE);
+      const bool self_cell = wv == 0 && c == 0;
+      const float shx = __shfl(my_sx, c, TFF_WAVE), shy = __shfl(my_sy, c, TFF_WAVE),
+                  shz = __shfl(my_sz, c, TFF_WAVE);
+      // this cell's first 64 atoms from the registers, then the next cell's load is issued (also
+      // when this cell is empty)
+      if (lane < nj_all) {
+        s_n[wv][lane] = make_double4(pre_x.x, pre_x.y, pre_x.z, (double)pre_i);
+        s_nf[wv][lane] = make_float4(pre_f.x + shx, pre_f.y + shy, pre_f.z + shz, pre_f.w);
+      }
+      tff_wave_sync_lds();
+      if (c + 1 < ncw) {
+        const int jsn = __shfl(my_js, c + 1, TFF_WAVE), njn = __shfl(my_nj, c + 1, TFF_WAVE);
+        if (lane < njn) {
+          pre_x = sxyz[jsn + lane];
+          pre_i = sorted_idx[jsn + lane];
+          if (g.pf) pre_f = sf[jsn + lane];
+        }
+      }
+      for (int j0 = 0; j0 < nj_all; j0 += TFF_WAVE) {
+        const int nj = nj_all - j0 < TFF_WAVE ? nj_all - j0 : TFF_WAVE;
+        if (j0 > 0) {
+          if (lane < nj) {
+            const double4 r = sxyz[js + j0 + lane];
+            s_n[wv][lane] = make_double4(r.x, r.y, r.z, (double)sorted_idx[js + j0 + lane]);
+            if (g.pf) {
+              const float4 f = sf[js + j0 + lane];
+              s_nf[wv][lane] = make_float4(f.x + shx, f.y + shy, f.z + shz, f.w);
             }
           }
-          push(pb, ok, ai, bj, lane, pairs, max_npairs, npairs);
+          tff_wave_sync_lds();
         }
+        const int total = ni * nj;
+        int a = lane / nj, b = lane - a * nj;
+        const int da = TFF_WAVE / nj, db = TFF_WAVE - da * nj;
+        for (int t0 = 0; t0 < total; t0 += TFF_WAVE) {
+          bool ok = false;
+          int ai = 0, bj = 0;
+          if (a < ni) {
+            const int pa = h0 + a, pb = j0 + b;  // positions within their cells
+            if (!self_cell || pa <= pb) {
+              const int4 hx = s_hx[a];
+              bool exact = true;
+              if (g.pf) {
+                const float4 fa = s_hf[a], fb = s_nf[wv][b];
+                ai = __float_as_int(fa.w);
+                bj = __float_as_int(fb.w);
+                const float ex = fa.x - fb.x, ey = fa.y - fb.y, ez = fa.z - fb.z;
+                const float d2f = ex * ex + ey * ey + ez * ez;
+                ok = d2f < g.lo;
+                exact = !(d2f < g.lo) && !(d2f > g.hi);
+              }
+              if (self_cell && pa == pb) {
+                const double4 ra = s_h[a];
+                ai = bj = (int)ra.w;
+                ok = include_self != 0;
+              } else if (exact) {
+                const double4 ra = s_h[a], rb = s_n[wv][b];
+                ai = (int)ra.w;
+                bj = (int)rb.w;
+                const double d2 = tff_dist2_d(box, ra.x - rb.x, ra.y - rb.y, ra.z - rb.z);
+                ok = d2 <= cutoff2;
+              }
+              if (ok && col) {
+                bool ex = (hx.x == bj) | (hx.y == bj) | (hx.z == bj) | (hx.w == bj);
+                if (hx.x == -2) ex = tff_is_excluded(ai, bj, row_ptr, col);
+                ok = !ex;
+              }
+            }
+          }
+          const unsigned long long m = __ballot(ok);
+          if (WRITE && ok) {
+            const long long row = pos + __popcll(m & ((1ull << lane) - 1ull));
+            if (row < max_npairs) {
+              longlong2 v;
+              v.x = ai > bj ? ai : bj;
+              v.y = ai > bj ? bj : ai;
+              reinterpret_cast<longlong2*>(pairs)[row] = v;
+            }
+          }
+          pos += __popcll(m);
+          a += da;
+          b += db;
+          if (b >= nj) {
+            b -= nj;
+            ++a;
+          }
+        }
+        tff_wave_sync_lds();  // before the next chunk restages s_n
       }
     }
   }
-  flush(pb, lane, pairs, max_npairs, npairs);
+  if (!WRITE && lane == 0) wcount[home * CP_WAVES + wv] = (int)pos;
 }
-
 bool choose_grid(const TffBox& box, double cutoff, long long n, CellGrid& g) {
   if (!box.periodic || !(cutoff > 0.0)) return false;
   double w[3];
@@ -297,6 +455,17 @@ bool choose_grid(const TffBox& box, double cutoff, long long n, CellGrid& g) {
     g.nc[0] = nc[0]; g.nc[1] = nc[1]; g.nc[2] = nc[2];
     g.reach = reach;
     g.ncell = nc[0] * nc[1] * nc[2];
+    g.pf = box.ortho && nc[0] >= 2 * reach + 3 && nc[1] >= 2 * reach + 3 && nc[2] >= 2 * reach + 3;
+    {
+      // float32 error of the squared distance: each displacement component within 6 u L (the two
+      // roundings of the wrapped coordinates, the shift, two subtractions), so d^2 within
+      // 21 u L d + 3 u d^2; the band is eight times that at d = cutoff
+      const double u = 5.9604644775390625e-8, Lm = fmax(box.b[0], fmax(box.b[4], box.b[8]));
+      const double band = u * (168.0 * Lm * cutoff + 24.0 * cutoff * cutoff);
+      g.lo = (float)(cutoff * cutoff - band);
+      g.hi = (float)(cutoff * cutoff + band);
+      for (int k = 0; k < 3; ++k) g.L[k] = (float)box.b[4 * k];
+    }
     return true;
   }
   return false;
@@ -326,6 +495,8 @@ struct NbWs {
   int* start;
   int* cursor;
   int* sidx;
+  int *wcount, *woff;  // per (home cell, wave) pair counts and their exclusive scan
+  float4* sf;          // prefilter copies of the sorted atoms
   size_t bytes;
 };
 
@@ -345,6 +516,9 @@ NbWs carve(void* base, long long n, int ncell) {
   w.count = (int*)take(sizeof(int) * (size_t)(ncell + 1));
   w.start = (int*)take(sizeof(int) * (size_t)(ncell + 1));
   w.cursor = (int*)take(sizeof(int) * (size_t)(ncell + 1));
+  w.wcount = (int*)take(sizeof(int) * ((size_t)ncell * CP_WAVES + 1));
+  w.woff = (int*)take(sizeof(int) * ((size_t)ncell * CP_WAVES + 1));
+  w.sf = (float4*)take(sizeof(float4) * (size_t)(n > 0 ? n : 1));
   w.bytes = off;
   return w;
 }
@@ -394,15 +568,19 @@ extern "C" int tff_build_neighbor_list(int dtype, const void* coords, long long 
   }
   long long* P = reinterpret_cast<long long*>(pairs);
   int* NP = reinterpret_cast<int*>(npairs);
-  {
-    long long n2 = 2 * max_npairs;
-    long long blocks = (n2 + 255) / 256;
+  // the count starts at zero; rows past the found count are set to -1 after the search (the found
+  // rows are written once)
+  hipLaunchKernelGGL(k_fill_pairs, dim3(1), dim3(64), 0, stream, P, 0LL, NP);
+  TFF_CHECK_LAUNCH("k_fill_pairs");
+  auto fill_tail = [&]() {
+    long long blocks = (2 * max_npairs + 255) / 256;
     if (blocks > 65535) blocks = 65535;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_fill_pairs, dim3((unsigned)blocks), dim3(256), 0, stream, P, n2, NP);
-    TFF_CHECK_LAUNCH("k_fill_pairs");
-  }
-  if (natoms == 0) return TFF_OK;
+    hipLaunchKernelGGL(k_fill_tail, dim3((unsigned)blocks), dim3(256), 0, stream, P, max_npairs, (const int*)NP);
+    TFF_CHECK_LAUNCH("k_fill_tail");
+    return TFF_OK;
+  };
+  if (natoms == 0) return fill_tail();
   const double cutoff2 = cutoff * cutoff;
   CellGrid g{{1, 1, 1}, 1, 0};
   bool use_cells = algorithm == TFF_NBLIST_CELL_LIST && choose_grid(box, cutoff, natoms, g);
@@ -417,20 +595,27 @@ extern "C" int tff_build_neighbor_list(int dtype, const void* coords, long long 
     return TFF_ERR_ARG;
   }
   TFF_CHECK_LAUNCH("k_promote");
-  if (!use_cells)
-    return launch_nsquared(ws.xyz, natoms, box, cutoff2, include_self, excl_row_ptr, excl_col, P, max_npairs,
-                           NP, stream);
+  if (!use_cells) {
+    const int rc = launch_nsquared(ws.xyz, natoms, box, cutoff2, include_self, excl_row_ptr, excl_col, P,
+                                   max_npairs, NP, stream);
+    return rc ? rc : fill_tail();
+  }
   const int m = g.ncell;
   hipLaunchKernelGGL(k_zero_int, dim3((m + 1 + 255) / 256), dim3(256), 0, stream, ws.count, m + 1);
   hipLaunchKernelGGL(k_zero_int, dim3((m + 1 + 255) / 256), dim3(256), 0, stream, ws.cursor, m + 1);
   hipLaunchKernelGGL(k_bin, dim3(nbk), dim3(256), 0, stream, ws.xyz, natoms, box, g, ws.cell_of, ws.count);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, ws.count, m, ws.start);
-  hipLaunchKernelGGL(k_scatter, dim3(nbk), dim3(256), 0, stream, ws.xyz, natoms, ws.cell_of, ws.start, ws.cursor,
-                     ws.sidx, ws.sxyz);
+  hipLaunchKernelGGL(k_scatter, dim3(nbk), dim3(256), 0, stream, ws.xyz, natoms, box, ws.cell_of, ws.start,
+                     ws.cursor, ws.sidx, ws.sxyz, ws.sf);
   void* tok = tff_prof_begin("nblist_cell_pairs", stream);
-  hipLaunchKernelGGL(k_cell_pairs, dim3((m + NB_WAVES - 1) / NB_WAVES), dim3(NB_BLOCK), 0, stream, ws.start,
-                     ws.sidx, ws.sxyz, g, box, cutoff2, include_self, excl_row_ptr, excl_col, P, max_npairs, NP);
+  hipLaunchKernelGGL(k_cell_pairs<false>, dim3(m), dim3(CP_WAVES * TFF_WAVE), 0, stream, ws.start, ws.sidx,
+                     ws.sxyz, (const float4*)ws.sf, g, box, cutoff2, include_self, excl_row_ptr, excl_col,
+                     ws.wcount, (const int*)ws.woff, P, max_npairs, NP);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, ws.wcount, m * CP_WAVES, ws.woff);
+  hipLaunchKernelGGL(k_cell_pairs<true>, dim3(m), dim3(CP_WAVES * TFF_WAVE), 0, stream, ws.start, ws.sidx,
+                     ws.sxyz, (const float4*)ws.sf, g, box, cutoff2, include_self, excl_row_ptr, excl_col,
+                     ws.wcount, (const int*)ws.woff, P, max_npairs, NP);
   tff_prof_end(tok, stream);
   TFF_CHECK_LAUNCH("k_cell_pairs");
-  return TFF_OK;
+  return fill_tail();
 }

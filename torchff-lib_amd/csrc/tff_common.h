@@ -31,6 +31,7 @@ struct TffBox {
   double inv[9];
   double det;
   int periodic;
+  int ortho;  // periodic with every off-diagonal entry of the box and of its inverse zero
 };
 
 // Host: cofactor inverse with the reference's formula, contraction off.
@@ -39,6 +40,7 @@ static inline TffBox tff_make_box(const double* m) {
   TffBox B;
   for (int k = 0; k < 9; ++k) B.b[k] = m ? m[k] : 0.0;
   B.periodic = m != nullptr;
+  B.ortho = 0;
   if (!m) {
     for (int k = 0; k < 9; ++k) B.inv[k] = 0.0;
     B.det = 0.0;
@@ -62,13 +64,28 @@ static inline TffBox tff_make_box(const double* m) {
   B.inv[3] = c01 * id; B.inv[4] = c11 * id; B.inv[5] = c21 * id;
   B.inv[6] = c02 * id; B.inv[7] = c12 * id; B.inv[8] = c22 * id;
   B.det = det;
+  B.ortho = 1;
+  for (int k = 0; k < 9; ++k)
+    if (k % 4 != 0 && (B.b[k] != 0.0 || B.inv[k] != 0.0)) B.ortho = 0;
   return B;
 }
 
-// Minimum-image displacement, float64, fixed operation order, no FMA.
+// Minimum-image displacement, float64, fixed operation order, no FMA.  Orthorhombic boxes take the
+// same operations without the zero off-diagonal terms: adding an exact zero leaves a sum unchanged
+// (up to the sign of a zero, which the squared distance loses), so the result is the general one.
 __device__ __forceinline__ void tff_min_image_d(const TffBox& B, double& x, double& y, double& z) {
 #pragma clang fp contract(off)
   if (!B.periodic) return;
+  if (B.ortho) {
+    double sa = x * B.inv[0], sb = y * B.inv[4], sc = z * B.inv[8];
+    sa = sa - round(sa);
+    sb = sb - round(sb);
+    sc = sc - round(sc);
+    x = sa * B.b[0];
+    y = sb * B.b[4];
+    z = sc * B.b[8];
+    return;
+  }
   double sa = x * B.inv[0] + y * B.inv[3] + z * B.inv[6];
   double sb = x * B.inv[1] + y * B.inv[4] + z * B.inv[7];
   double sc = x * B.inv[2] + y * B.inv[5] + z * B.inv[8];
@@ -90,6 +107,16 @@ __device__ __forceinline__ double tff_dist2_d(const TffBox& B, double dx, double
 template <typename T>
 __device__ __forceinline__ void tff_min_image(const TffBox& B, T& x, T& y, T& z) {
   if (!B.periodic) return;
+  if (B.ortho) {  // the general form without its zero terms (same values)
+    T sa = x * (T)B.inv[0], sb = y * (T)B.inv[4], sc = z * (T)B.inv[8];
+    sa -= rint(sa);
+    sb -= rint(sb);
+    sc -= rint(sc);
+    x = sa * (T)B.b[0];
+    y = sb * (T)B.b[4];
+    z = sc * (T)B.b[8];
+    return;
+  }
   T sa = x * (T)B.inv[0] + y * (T)B.inv[3] + z * (T)B.inv[6];
   T sb = x * (T)B.inv[1] + y * (T)B.inv[4] + z * (T)B.inv[7];
   T sc = x * (T)B.inv[2] + y * (T)B.inv[5] + z * (T)B.inv[8];
