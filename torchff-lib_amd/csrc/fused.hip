@@ -592,29 +592,10 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
 __global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ ncl_ptr, const int* __restrict__ cnt,
                                                       int* __restrict__ start, int* __restrict__ ncp,
                                                       int* __restrict__ status) {
-  __shared__ int s[1024];
-  const int m = *ncl_ptr;
-  int t = threadIdx.x;
-  int per = (m + 1023) / 1024;
-  int b = t * per, e = b + per < m ? b + per : m;
-  int sum = 0;
-  for (int k = b; k < e; ++k) sum += cnt[k];
-  s[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += v;
-    __syncthreads();
-  }
-  int run = t > 0 ? s[t - 1] : 0;
-  for (int k = b; k < e; ++k) {
-    start[k] = run;
-    run += cnt[k];
-  }
-  if (t == 1023) {
-    *ncp = s[1023];
-    status[1] = s[1023];
+  const int total = tff_block_scan_1024(cnt, *ncl_ptr, start, false);  // start holds ncl entries
+  if (threadIdx.x == 0) {
+    *ncp = total;
+    status[1] = total;
   }
 }
 

@@ -188,26 +188,7 @@ __global__ void k_bin(const double4* __restrict__ xyz, long long n, TffBox box, 
 
 // Exclusive scan of count[0..m) into start[0..m], single workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ count, int m, int* __restrict__ start) {
-  __shared__ int s_tot[1024];
-  int t = threadIdx.x;
-  int per = (m + 1023) / 1024;
-  int b = t * per, e = b + per < m ? b + per : m;
-  int sum = 0;
-  for (int k = b; k < e; ++k) sum += count[k];
-  s_tot[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? s_tot[t - off] : 0;
-    __syncthreads();
-    s_tot[t] += v;
-    __syncthreads();
-  }
-  int run = t > 0 ? s_tot[t - 1] : 0;
-  for (int k = b; k < e; ++k) {
-    start[k] = run;
-    run += count[k];
-  }
-  if (t == 1023) start[m] = s_tot[1023];
+  tff_block_scan_1024(count, m, start, true);
 }
 
 // also the prefilter's float32 copy (orthorhombic boxes): coordinates wrapped into the box, the atom
