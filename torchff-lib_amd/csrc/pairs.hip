@@ -28,7 +28,9 @@
 // and a wave issues them transposed: it compacts the atoms it updates into LDS
 // and eight adjacent lanes write one atom's line.  Consecutive rows that share
 // an atom (the i side of a cell-list pair list) are summed first by a
-// segmented scan, so such a run costs one line.  The integer sums hold up to
+// segmented scan, so such a run costs one line; the j side goes through a
+// per-wave LDS hash of atoms (exact integer words), so each distinct atom of a
+// wave step costs one line.  The integer sums hold up to
 // 2^39 per atom; contributions (run sums) of magnitude 2^30 or more
 // (near-overlapping atoms) go to float64 atomics instead, so an atom's integer
 // sum would need 512 such contributions to overflow.  The
@@ -91,9 +93,9 @@ __device__ __forceinline__ bool split_words(double v, long long& h, long long& l
 // Lanes with adds set add their W words w[] into rec[key * W + c]: the lanes are compacted into the
 // wave's LDS area (s_w: 64 * W words, s_k: 64 keys) and the words go out 64 / W atoms per
 // instruction, W adjacent lanes per atom.  Wave-uniform control flow.
-template <int W>
+template <int W, typename K>
 __device__ __forceinline__ void wave_flush(bool adds, long long key, const long long (&w)[W],
-                                           long long* __restrict__ rec, long long* s_w, long long* s_k) {
+                                           long long* __restrict__ rec, long long* s_w, K* s_k) {
   const unsigned long long m = __ballot(adds);
   if (m == 0) return;
   const int lane = threadIdx.x & (TFF_WAVE - 1);
@@ -102,7 +104,7 @@ __device__ __forceinline__ void wave_flush(bool adds, long long key, const long 
     const int r = __popcll(m & ((1ull << lane) - 1ull));
 #pragma unroll
     for (int c = 0; c < W; ++c) s_w[r * W + c] = w[c];
-    s_k[r] = key;
+    s_k[r] = (K)key;
   }
   tff_wave_sync_lds();
   constexpr int KPI = TFF_WAVE / W;
@@ -110,7 +112,7 @@ __device__ __forceinline__ void wave_flush(bool adds, long long key, const long 
     const int idx = base + lane / W;
     if (idx < n) {
       const long long v = s_w[base * W + lane];
-      if (v != 0) atomicAdd((unsigned long long*)(rec + s_k[idx] * W + (lane % W)), (unsigned long long)v);
+      if (v != 0) atomicAdd((unsigned long long*)(rec + (long long)s_k[idx] * W + (lane % W)), (unsigned long long)v);
     }
   }
   tff_wave_sync_lds();
@@ -120,9 +122,9 @@ __device__ __forceinline__ void wave_flush(bool adds, long long key, const long 
 // nothing).  Runs of lanes with the same key (consecutive rows of one atom) are summed by a
 // segmented scan and the run's last lane adds the sum.  Inactive lanes add zero and do not break
 // a run.  Wave-uniform control flow.
-template <int NC>
+template <int NC, typename K>
 __device__ __forceinline__ void wave_accum(bool act, long long key, const double (&v)[NC], unsigned chmask,
-                                           const Acc& acc, long long* s_w, long long* s_k) {
+                                           const Acc& acc, long long* s_w, K* s_k) {
   if (__ballot(act) == 0) return;
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const long long prev = __shfl_up(key, 1, TFF_WAVE);
@@ -176,6 +178,77 @@ __device__ __forceinline__ void wave_accum(bool act, long long key, const double
   }
 }
 
+// Per-wave LDS hash of the atoms one wave step updates on the j side (a cell-list pair list repeats
+// each j a few times within 64 rows): key[64] (atom index, -1 empty) and per slot HW int64 words
+// (x, y, z, q high then low), exact integer sums (order independent).  At most 64 keys per step,
+// so the table never fills.  The flush reuses the table's memory.  (The sigma / epsilon gradient
+// channels, rarely requested, take the direct path.)
+constexpr int HW = 8;
+struct WaveHash {
+  int* key;
+  long long* w;  // 64 * HW
+};
+
+__device__ __forceinline__ void hash_clear(const WaveHash& h) {
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  h.key[lane] = -1;
+  longlong2* wv = reinterpret_cast<longlong2*>(h.w + lane * HW);
+#pragma unroll
+  for (int c = 0; c < HW / 2; ++c) wv[c] = make_longlong2(0, 0);
+  tff_wave_sync_lds();
+}
+
+// adds v[c] (channels in chmask) of this lane to atom key's slot; contributions too large for the
+// integer words go to the float64 fallback of the atom directly
+template <int NC>
+__device__ __forceinline__ void hash_add(const WaveHash& h, bool act, int key, const double (&v)[NC],
+                                         unsigned chmask, const Acc& acc) {
+  if (!act) return;
+  int p = (int)(((unsigned)key * 2654435761u) >> 26);
+  int slot = -1;
+  for (int t = 0; t < TFF_WAVE; ++t) {
+    const int old = atomicCAS(&h.key[p], -1, key);
+    if (old == -1 || old == key) {
+      slot = p;
+      break;
+    }
+    p = (p + 1) & (TFF_WAVE - 1);
+  }
+  if (slot < 0) {  // not reached: at most 64 keys per step
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if ((chmask >> c) & 1u) atomicAdd(acc.ovf + (long long)key * NCH + c, v[c]);
+    return;
+  }
+  long long* w = h.w + slot * HW;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (!((chmask >> c) & 1u)) continue;
+    long long hi, lo;
+    if (!split_words(v[c], hi, lo)) {
+      atomicAdd(acc.ovf + (long long)key * NCH + c, v[c]);
+      continue;
+    }
+    if (hi) atomicAdd((unsigned long long*)(w + c), (unsigned long long)hi);
+    if (lo) atomicAdd((unsigned long long*)(w + 4 + c), (unsigned long long)lo);
+  }
+}
+
+// every filled slot's words to its atom's lines (transposed flush), then the table is free
+__device__ __forceinline__ void hash_flush(const WaveHash& h, unsigned chmask, const Acc& acc) {
+  const int lane = threadIdx.x & (TFF_WAVE - 1);
+  tff_wave_sync_lds();
+  const int key = h.key[lane];
+  long long wa[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) wa[c] = h.w[lane * HW + c];
+  tff_wave_sync_lds();  // the table's memory is reused by the flush
+  bool any = false;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) any |= wa[c] != 0;
+  wave_flush<8>(key >= 0 && any, key, wa, acc.a, h.w, h.key);
+}
+
 // Adds v[c] of the lanes with act set into this wave's LDS slot tab[2 key + c] (wave-uniform
 // order: the keys in order of their first lane, each key's sum by a butterfly); direct float64
 // atomics into ovf[2 key + c] when the table is too large for LDS.
@@ -217,8 +290,8 @@ __global__ __launch_bounds__(PB) void k_pair_terms(const T* __restrict__ coords,
                                                    T* __restrict__ g_eps_pairs) {
   constexpr int TAB_WORDS = MODE == TFF_PAIR_LJ_TYPES ? 2 * TAB_MAX : 1;
   __shared__ double s_tab[PB / TFF_WAVE][TAB_WORDS];
-  __shared__ long long s_flush_w[PB / TFF_WAVE][TFF_WAVE * 8];
-  __shared__ long long s_flush_k[PB / TFF_WAVE][TFF_WAVE];
+  __shared__ __align__(16) long long s_hash_w[PB / TFF_WAVE][TFF_WAVE * HW];
+  __shared__ int s_hash_k[PB / TFF_WAVE][TFF_WAVE];
   const long long ntab = MODE == TFF_PAIR_LJ_TYPES ? a.ntypes * a.ntypes : 0;
   const bool tab_lds = MODE == TFF_PAIR_LJ_TYPES && tmask && ntab <= TAB_MAX;
   if (tab_lds) {
@@ -346,10 +419,16 @@ __global__ __launch_bounds__(PB) void k_pair_terms(const T* __restrict__ coords,
       }
     }
     if (chmask) {
-      long long* s_w = s_flush_w[threadIdx.x / TFF_WAVE];
-      long long* s_k = s_flush_k[threadIdx.x / TFF_WAVE];
-      wave_accum<NCH>(act, valid ? i : -1, vi, chmask, atoms, s_w, s_k);
-      wave_accum<NCH>(act, valid ? j : -1, vj, chmask, atoms, s_w, s_k);
+      // i side: runs of one atom summed by a segmented scan, flushed directly; j side: through the
+      // wave's hash (both use the hash memory in turn)
+      const WaveHash wh{s_hash_k[threadIdx.x / TFF_WAVE], s_hash_w[threadIdx.x / TFF_WAVE]};
+      wave_accum<NCH>(act, valid ? i : -1, vi, chmask, atoms, wh.w, wh.key);
+      if (chmask & 15u) {
+        hash_clear(wh);
+        hash_add<NCH>(wh, act && valid, (int)j, vj, chmask, atoms);
+        hash_flush(wh, chmask, atoms);
+      }
+      if (chmask & 48u) wave_accum<NCH>(act, valid ? j : -1, vj, chmask & 48u, atoms, wh.w, wh.key);
     }
     if (MODE == TFF_PAIR_LJ_TYPES && tmask)
       wave_accum_table(act, tix, vt, tmask, tab_lds ? s_tab[threadIdx.x / TFF_WAVE] : nullptr, tab_ovf);
