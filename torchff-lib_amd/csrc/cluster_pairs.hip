@@ -115,29 +115,10 @@ __global__ void k_cp_keys(const T* __restrict__ coords, CpGeom g, int* __restric
   atomicAdd(&count[cell], 1);
 }
 
-// exclusive scan of cnt[0..m) into off[0..m], one workgroup of 1024 threads, each a contiguous chunk
+// exclusive scan of cnt[0..m) into off[0..m] (off[m] = total), one workgroup of 1024 threads
 template <typename C, typename O>
 __global__ __launch_bounds__(1024) void k_cp_scan(const C* __restrict__ cnt, long long m, O* __restrict__ off) {
-  __shared__ O s_tot[1024];
-  const int t = threadIdx.x;
-  const long long per = (m + 1023) / 1024;
-  const long long b = t * per, e = b + per < m ? b + per : m;
-  O sum = 0;
-  for (long long k = b; k < e; ++k) sum += (O)cnt[k];
-  s_tot[t] = sum;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const O v = t >= o ? s_tot[t - o] : (O)0;
-    __syncthreads();
-    s_tot[t] += v;
-    __syncthreads();
-  }
-  O run = t > 0 ? s_tot[t - 1] : (O)0;
-  for (long long k = b; k < e; ++k) {
-    off[k] = run;
-    run += (O)cnt[k];
-  }
-  if (t == 1023) off[m] = s_tot[1023];
+  tff_block_scan_1024<C, O>(cnt, m, off, true);
 }
 
 __global__ void k_cp_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ start,

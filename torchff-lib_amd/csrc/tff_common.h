@@ -171,31 +171,32 @@ __device__ __forceinline__ double tff_block_sum(double v) {
 // Exclusive scan of in[0..m) into out[0..m) (and out[m] = total when write_total) by one workgroup
 // of 1024 threads, in tiles of 4096: each thread loads four consecutive entries (coalesced), wave
 // scans by shuffles, wave totals through LDS, a running carry between tiles.  Every thread of the
-// workgroup calls it; returns the total in every thread.
-__device__ __forceinline__ int tff_block_scan_1024(const int* __restrict__ in, int m, int* __restrict__ out,
-                                                   bool write_total) {
-  __shared__ int s_w[1024 / TFF_WAVE];
-  __shared__ int s_carry;
+// workgroup calls it; returns the total in every thread.  O: the type of the sums.
+template <typename C, typename O>
+__device__ __forceinline__ O tff_block_scan_1024(const C* __restrict__ in, long long m, O* __restrict__ out,
+                                                 bool write_total) {
+  __shared__ O s_w[1024 / TFF_WAVE];
+  __shared__ O s_carry;
   const int t = threadIdx.x, lane = t & (TFF_WAVE - 1), wv = t / TFF_WAVE;
   if (t == 0) s_carry = 0;
   __syncthreads();
-  for (int base = 0; base < m; base += 4096) {
-    const int i0 = base + 4 * t;
-    int v[4];
+  for (long long base = 0; base < m; base += 4096) {
+    const long long i0 = base + 4 * t;
+    O v[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < m ? in[i0 + k] : 0;
-    const int local = v[0] + v[1] + v[2] + v[3];
-    int incl = local;
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < m ? (O)in[i0 + k] : (O)0;
+    const O local = v[0] + v[1] + v[2] + v[3];
+    O incl = local;
 #pragma unroll
     for (int off = 1; off < TFF_WAVE; off <<= 1) {
-      const int x = __shfl_up(incl, off, TFF_WAVE);
+      const O x = __shfl_up(incl, off, TFF_WAVE);
       if (lane >= off) incl += x;
     }
     if (lane == TFF_WAVE - 1) s_w[wv] = incl;
     __syncthreads();
-    int before = s_carry;
+    O before = s_carry;
     for (int q = 0; q < wv; ++q) before += s_w[q];
-    int run = before + incl - local;
+    O run = before + incl - local;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (i0 + k < m) out[i0 + k] = run;
@@ -205,7 +206,7 @@ __device__ __forceinline__ int tff_block_scan_1024(const int* __restrict__ in, i
     if (t == 1023) s_carry = before + incl;
     __syncthreads();
   }
-  const int total = s_carry;
+  const O total = s_carry;
   if (write_total && t == 0) out[m] = total;
   return total;
 }
