@@ -50,6 +50,15 @@ int tff_prof_collect(const char* name, double* total_ms, long long* count);
  * pair search, then the FFT and the gather on an auxiliary stream concurrently with the pair
  * search and pair kernel; 3: the whole chain on the auxiliary stream.  Other values mean 1. */
 void tff_fused_set_overlap(int mode);
+/* Graph replay of tff_fused_energy_forces (no counterpart in the reference; results are identical
+ * either way).  The launches of a call up to its last kernel are captured once into a HIP graph
+ * per device, input and workspace addresses, box and parameters (up to 8 kept per device, the
+ * least recently used dropped after a device synchronisation), and replayed by later calls with
+ * the same key; the last kernel, which writes forces_out, energies_out and the status words, is
+ * launched per call.  mode 1 (default): for float32 arithmetic below 300 000 atoms (where replay
+ * measured faster); 2: always; 0: never.  A call whose stream is being captured, or made while
+ * tff_prof_enable(1) is on, launches its kernels one by one.  Other values mean 1. */
+void tff_fused_set_graphs(int mode);
 
 /* --------------------------------------------------------- neighbor list */
 #define TFF_NBLIST_NSQUARED 0
@@ -267,8 +276,9 @@ size_t tff_fused_workspace_bytes(int compute_dtype, long long natoms, const doub
  * cutoff on every call).  box_device (optional, io_dtype, 9 values): the caller's device copy
  * of the box; the call flags TFF_FUSED_BOX_STALE when it differs from box_host.  status_host
  * (optional, 2 int32 in pinned host memory mapped for the device): the last kernel of the call
- * copies status_out there, so the caller can read the flags once the call's work has completed
- * (an event) without a device-to-host copy.  Moduli as for tff_pme_energy with zmoduli of length
+ * copies status_out there, word 1 first and word 0 after a system-scope fence, so the caller can
+ * read the flags without a device-to-host copy (they are final when the last kernel starts: a
+ * caller that stores -1 in word 0 before the call reads valid flags once word 0 is not -1).  Moduli as for tff_pme_energy with zmoduli of length
  * K2/2 + 1, in compute_dtype. */
 int tff_fused_energy_forces(int io_dtype, int compute_dtype, const void* coords, const void* charges,
                             const void* sigma, const void* epsilon, long long natoms, const double* box_host,

@@ -140,3 +140,86 @@ def test_fused_tiny_grid(precision, te, tf):
     scale = sum(abs(v) for v in terms.values())
     assert abs(Ed - E) < te * scale, (Ed, E)
     assert _rms_rel(Fd, F) < tf
+
+
+def _tensors(w, dev="cuda"):
+    return [torch.tensor(w[k], dtype=torch.float64, device=dev)
+            for k in ("coords", "box", "charges", "sigma", "epsilon")]
+
+
+def _same(a, b):
+    assert float(a[0]) == float(b[0])
+    assert torch.equal(a[1], b[1])
+    assert torch.equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_fused_graph_replay_matches_one_by_one(precision):
+    """The forward launches captured once into a graph and replayed give, bitwise, what the same
+    kernels launched one by one give: with the coordinates changed in place between calls (same
+    addresses, so the same graph), with other work and synchronisations between replays, and
+    after a box change (a new key, so a new capture)."""
+    from torchff import _lib as L
+    from torchff.fused import for_workload
+    lib = L.load()
+    w = make_workload("water900")
+    m = for_workload(w, precision)
+    t = _tensors(w)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    try:
+        for step in range(4):
+            if step:
+                t[0].add_(0.01 * torch.randn(t[0].shape, generator=gen, device="cuda", dtype=torch.float64))
+            lib.tff_fused_set_graphs(2)
+            g1 = m(*t)  # the first call of a key captures (then launches the graph), later ones replay
+            g2 = m(*t)
+            g3 = m(*t)
+            torch.randn(4096, device="cuda").sum().item()  # other work and a synchronisation
+            lib.tff_fused_set_graphs(0)
+            e = m(*t)
+            for g in (g1, g2, g3):
+                _same(g, e)
+        t[1].mul_(1.01)
+        t[0].mul_(1.01)
+        lib.tff_fused_set_graphs(2)
+        g = m(*t)
+        g2 = m(*t)
+        lib.tff_fused_set_graphs(0)
+        e = m(*t)
+        _same(g, e)
+        _same(g2, e)
+    finally:
+        lib.tff_fused_set_graphs(1)
+    m.check()
+
+
+def test_fused_inside_torch_graph_capture():
+    """A forward call captured into a torch CUDA graph (the library then launches its kernels one by
+    one into the capture) replays to the one-by-one results of the same inputs, bitwise, with the
+    coordinates changed in place between replays; check() reads the last replay's status."""
+    from torchff.fused import for_workload
+    w = make_workload("water900")
+    m = for_workload(w, "mixed")
+    t = _tensors(w)
+    ref = for_workload(w, "mixed")
+    m(*t)  # eager first call: workspace, status ring and box cache
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        m(*t)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = m(*t)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    for rep in range(3):
+        if rep:
+            t[0].add_(0.01 * torch.randn(t[0].shape, generator=gen, device="cuda", dtype=torch.float64))
+        torch.randn(4096, device="cuda").sum().item()
+        g.replay()
+        torch.cuda.synchronize()
+        _same(out, ref(*t))
+    m.check()
+    ref.check()

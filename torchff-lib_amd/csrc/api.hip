@@ -59,6 +59,11 @@ void tff_prof_end(void* token, hipStream_t stream) {
   if (k < g_prof.size()) hipEventRecord(g_prof[k].b, stream);
 }
 
+bool tff_prof_active() {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  return g_prof_on;
+}
+
 extern "C" const char* tff_last_error(void) { return g_err; }
 
 extern "C" int tff_abi_version(void) { return 2; }

@@ -233,6 +233,7 @@ void tff_set_error(const char* fmt, ...);
 // Optional event timing around one kernel launch (no-ops unless tff_prof_enable(1)).
 void* tff_prof_begin(const char* name, hipStream_t stream);
 void tff_prof_end(void* token, hipStream_t stream);
+bool tff_prof_active();  // event timing requested (tff_prof_enable)
 
 #define TFF_CHECK_LAUNCH(name)                                              \
   do {                                                                      \

@@ -31,6 +31,7 @@ _SIGNATURES = {
     "tff_prof_enable": (None, [_i]),
     "tff_prof_collect": (_i, [ctypes.c_char_p, ctypes.POINTER(_d), ctypes.POINTER(_ll)]),
     "tff_fused_set_overlap": (None, [_i]),
+    "tff_fused_set_graphs": (None, [_i]),
     "tff_nblist_workspace_bytes": (_sz, [_ll, _vp, _d, _i]),
     "tff_build_neighbor_list": (_i, [_i, _vp, _ll, _vp, _d, _i, _i, _vp, _vp, _vp, _ll, _vp, _vp, _sz, _vp]),
     "tff_pair_workspace_bytes": (_sz, [_i, _ll, _ll, _ll]),

@@ -21,6 +21,7 @@ cluster pair list; it is not itself differentiable.
 """
 
 import math
+import time
 from collections import deque
 
 import torch
@@ -88,12 +89,16 @@ class FusedNonbondedPME(nn.Module):
         self._ws_key = None
         self._mod = None
         self.status = None
-        # status words of recent calls, mirrored into pinned host memory by each call's last kernel:
-        # (event, host tensor, what) in call order; read once the event has completed, so that an
-        # overflow surfaces on a later call (or at check()) without synchronising the call itself
+        # status words of recent calls, mirrored into a row of pinned host memory by each call's last
+        # kernel (word 1, a fence, then word 0 over the -1 the host stored before the call):
+        # (row, what) in call order; read once word 0 has arrived, so that an overflow surfaces on a
+        # later call (or at check()) without synchronising the call itself or recording an event
         self._ring = None
+        self._ring_np = None
         self._ring_next = 0
+        self._ring_dev = None
         self._pending = deque()
+        self._captured = False  # a call was captured into a CUDA graph (its replays write ring row _RING)
 
     def _workspace(self, box_host, device):
         lib = L.load()
@@ -109,30 +114,60 @@ class FusedNonbondedPME(nn.Module):
 
     def _status_slot(self, device):
         """(device status tensor, pinned host mirror) for a new call, after reading the status of
-        every earlier call that has completed (raising if one of them was invalid)."""
-        self._poll(block=len(self._pending) >= _RING)
+        every earlier call that has completed (raising if one of them was invalid).  A call made
+        while the stream is being captured into a CUDA graph takes the ring's last row, which its
+        replays rewrite and check() reads (events cannot be polled under capture)."""
+        capturing = torch.cuda.is_current_stream_capturing()
+        if not capturing:
+            self._poll(block=len(self._pending) >= _RING)
         if self._ring is None:
-            self._ring = torch.zeros((_RING, 2), dtype=torch.int32, pin_memory=True)
-        host = self._ring[self._ring_next]
-        self._ring_next = (self._ring_next + 1) % _RING
-        return torch.empty(2, dtype=torch.int32, device=device), host
+            # pinned memory is allocated outside any capture: the first call of a module is eager
+            self._ring = torch.full((_RING + 1, 2), -1, dtype=torch.int32, pin_memory=True)
+            self._ring_np = self._ring.numpy()
+            self._ring_dev = device
+        if capturing:
+            row = _RING
+            self._captured = True
+        else:
+            row = self._ring_next
+            self._ring_next = (row + 1) % _RING
+        self._ring_np[row] = -1
+        return torch.empty(2, dtype=torch.int32, device=device), self._ring[row], row
 
-    def _record(self, host, what, stream_tensor):
-        ev = torch.cuda.Event()
-        ev.record(torch.cuda.current_stream(stream_tensor.device))
-        self._pending.append((ev, host, what))
+    def _record(self, row, what):
+        if not torch.cuda.is_current_stream_capturing():
+            self._pending.append((row, what))
+
+    def _wait_row(self, row):
+        """Flags of ring row ``row`` once its call's last kernel has written them: a short spin,
+        then sleeps; after a second, a device synchronisation (which raises on a device error)."""
+        t0 = time.perf_counter()
+        spins = 0
+        while True:
+            flags = int(self._ring_np[row, 0])
+            if flags >= 0:
+                return flags
+            spins += 1
+            if spins > 64:
+                time.sleep(2e-5)
+            if time.perf_counter() - t0 > 1.0:
+                torch.cuda.synchronize(self._ring_dev)
+                flags = int(self._ring_np[row, 0])
+                if flags < 0:
+                    raise RuntimeError("FusedNonbondedPME: a call's status words never arrived")
+                return flags
 
     def _poll(self, block=False, all_=False):
         """Read the status of completed calls in call order; raise on the first invalid one.
         block: wait for the oldest call; all_: wait for every call."""
         while self._pending:
-            ev, host, what = self._pending[0]
-            if all_ or (block and len(self._pending) >= _RING):
-                ev.synchronize()
-            elif not ev.query():
-                break
+            row, what = self._pending[0]
+            flags = int(self._ring_np[row, 0])
+            if flags < 0:
+                if not (all_ or (block and len(self._pending) >= _RING)):
+                    break
+                flags = self._wait_row(row)
             self._pending.popleft()
-            flags = int(host[0])
             if flags & INVALID:
                 self._pending.clear()
                 raise RuntimeError(f"FusedNonbondedPME: an earlier {what} returned invalid results: "
@@ -180,7 +215,7 @@ class FusedNonbondedPME(nn.Module):
         rp, col = self.excl_row_ptr, self.excl_col_indices
         out = torch.empty(5, dtype=torch.float64, device=dev)
         forces = torch.empty_like(c)
-        status, host = self._status_slot(dev)
+        status, host, row = self._status_slot(dev)
         bd = self._box_device(box, io)
         cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
         rc = lib.tff_fused_energy_forces(L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms,
@@ -189,7 +224,7 @@ class FusedNonbondedPME(nn.Module):
                                          L.ptr(out), L.ptr(forces), L.ptr(status), L.ptr(bd), L.ptr(host),
                                          L.ptr(ws), ws.numel(), L.stream_of(c))
         L.check(rc, "FusedNonbondedPME")
-        self._record(host, "forward call", c)
+        self._record(row, "forward call")
         self.status = status
         if check:
             self.check()
@@ -214,7 +249,7 @@ class FusedNonbondedPME(nn.Module):
         gf = None if grad_forces is None else grad_forces.detach().to(device=dev, dtype=io).contiguous()
         outs = [torch.empty_like(c) if needs[0] else None] + \
                [torch.empty(self.natoms, dtype=io, device=dev) if n else None for n in needs[1:]]
-        status, host = self._status_slot(dev)
+        status, host, row = self._status_slot(dev)
         bd = self._box_device(box, io)
         rp, col = self.excl_row_ptr, self.excl_col_indices
         rc = lib.tff_fused_energy_forces_backward(
@@ -223,7 +258,7 @@ class FusedNonbondedPME(nn.Module):
             L.ptr(ge), L.ptr(gf), *(L.ptr(o) for o in outs), L.ptr(status), L.ptr(bd), L.ptr(host),
             L.ptr(self._bws), self._bws.numel(), L.stream_of(c))
         L.check(rc, "FusedNonbondedPME backward")
-        self._record(host, "backward call", c)
+        self._record(row, "backward call")
         self.backward_status = status
         return outs
 
@@ -233,14 +268,19 @@ class FusedNonbondedPME(nn.Module):
 
     def check(self):
         """Wait for every outstanding call (forward and backward) and raise if one of them
-        overflowed an internal capacity or met a stale box cache."""
+        overflowed an internal capacity or met a stale box cache.  Calls captured into a CUDA
+        graph report the status of their most recent replay (after a device synchronisation)."""
         while self._pending:
-            ev, host, what = self._pending.popleft()
-            ev.synchronize()
-            flags = int(host[0])
+            row, what = self._pending.popleft()
+            flags = self._wait_row(row)
             if flags:
                 self._pending.clear()
                 raise RuntimeError(f"FusedNonbondedPME: {what}: {status_message(flags)}")
+        if self._captured:
+            torch.cuda.synchronize(self._ring_dev)
+            flags = int(self._ring_np[_RING, 0])
+            if flags > 0:  # -1: not replayed yet
+                raise RuntimeError(f"FusedNonbondedPME: a replay of a captured call: {status_message(flags)}")
 
 
 class _FusedEnergyForces(torch.autograd.Function):
