@@ -2525,13 +2525,13 @@ extern "C" int tff_fused_energy_forces_backward(
 
 namespace {
 // ---------------------------------------------------------------- graph replay of the forward core
-// The forward core (21 launches on two streams) is captured once per key into a HIP graph and
+// The forward core (19 kernels on two streams) is captured once per key into a HIP graph and
 // replayed; k_finalize, which writes the caller's outputs, is launched after the replay, so every
 // call still returns its own output buffers.  A key holds everything the captured launches read:
 // the device, the input and workspace addresses, the box, every scalar parameter and the stream
 // schedule.  Replays read the inputs at their addresses when they run, so coordinates updated in
-// place are picked up.  Measured on the 96,000-atom water box (mixed): 0.285 ms per evaluation
-// replayed against 0.305 ms launched one kernel at a time (the same kernels, bitwise equal results).
+// place are picked up.  Launched one by one, the pair search waits behind the reciprocal-space
+// kernels of the other stream; the same kernels replayed give bitwise equal results.
 constexpr int GRAPH_CACHE = 8;  // captured forward cores kept per device (least recently used evicted)
 
 struct FusedKey {
@@ -2556,8 +2556,8 @@ hipStream_t g_capture_stream[64];
 // a graph is captured and launched for a call on st: by the mode (0: never; 1: float32 arithmetic
 // below SPREAD_FIRST_ATOMS atoms; 2: always), no event timing requested, st not itself being
 // captured (an enclosing capture records the kernels one by one instead).  Mode 1 follows the
-// measurements per evaluation (graph against one by one): 96k atoms mixed 0.297 against 0.311 ms,
-// fp64 0.780 against 0.767, 999k atoms mixed 2.352 against 2.342.
+// measurements per evaluation (graph against one by one): 96k atoms mixed 0.294 against 0.304 ms,
+// fp64 0.773 against 0.761, 999k atoms mixed 2.350 against 2.351.
 bool graphs_usable(hipStream_t st, bool f32, long long n) {
   const int mode = g_graphs_mode.load();
   if (mode == 0 || (mode == 1 && !(f32 && n < SPREAD_FIRST_ATOMS)) || tff_prof_active()) return false;
