@@ -87,6 +87,30 @@ def test_graph_replays_match_direct_launches():
             assert np.array_equal(f, r.forces), (rep, r.rid)
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_results_stored_to_host_match_device_output(use_graph):
+    """The request path's results, stored into pinned host memory by the last kernel, equal bit for
+    bit those of the same uploaded batch evaluated into device memory (the device-resident path),
+    over two batches in a row."""
+    eng = ServingEngine(slots=1, max_k=32, use_graph=use_graph)
+    s = eng.slots[0]
+    for seed in (8, 9):
+        for r in request_stream(16, min_waters=20, max_waters=600, seed=seed):
+            assert s.try_add(r)
+        f0, nsys = 5 * eng.caps["max_systems"], len(s.requests)
+        s.launch()
+        s.event.synchronize()
+        host = s.h_out.clone()
+        with torch.cuda.stream(s.stream):
+            s.run()  # the same uploaded batch, results into d_out
+        s.stream.synchronize()
+        dev = s.d_out.cpu()
+        # the written parts: 5 energies per system, 3 forces per atom
+        assert torch.equal(host[:5 * nsys], dev[:5 * nsys]), seed
+        assert torch.equal(host[f0: f0 + 3 * s.natoms], dev[f0: f0 + 3 * s.natoms]), seed
+        s.finish(0.0)
+
+
 def test_serve_stream_realtime():
     reqs = request_stream(200, rate=2000.0, min_waters=20, max_waters=1000, seed=7)
     eng = ServingEngine(slots=2, max_k=32)

@@ -469,12 +469,14 @@ __global__ __launch_bounds__(RW * TFF_WAVE) void k_serve_gather(const BatchHead*
         ek = tff_wave_sum(ek);
         q2 = tff_wave_sum(q2);
         if (lane == 0) {
+          // out may be pinned host memory: stores only, no read-back
           double* o = out + 5 * (size_t)s;
+          const double erec = coul * ek, eself = -coul * sd.alpha * 0.5641895835477563 * q2;
           o[0] = er;
           o[1] = 0.0;
-          o[2] = coul * ek;
-          o[3] = -coul * sd.alpha * 0.5641895835477563 * q2;
-          o[4] = o[0] + o[2] + o[3];
+          o[2] = erec;
+          o[3] = eself;
+          o[4] = er + erec + eself;
         }
       }
       continue;  // uniform: every wave of the workgroup takes this branch

@@ -9,7 +9,10 @@ into a HIP graph and replayed for every batch:
 * a **slot** owns a stream, a device workspace sized by fixed capacities
   (systems, 64-atom tasks, FFT tiles per pass, largest grid edge, grid points,
   atoms), pinned host staging for inputs / outputs, the host table of the
-  batch, and the captured graph;
+  batch, and two captured graphs: one whose last kernel stores the results
+  straight into the pinned output block (requests: no copy back, whose wait
+  after a replay measured 25-50 us), one writing them to device memory
+  (device-resident consumers such as the MD model);
 * **admission by HBM footprint**: a request joins the batch being formed only
   if its atoms, grid points, tasks and tiles fit what is left of the slot's
   capacities (``tff_serve_table_add`` refuses it otherwise); the capacities
@@ -143,16 +146,20 @@ class _Slot:
         # the pinned table comes uninitialised (possibly a reused block): clear its head now
         self.reset()
         self.event = torch.cuda.Event()
-        self.graph = None
+        self.graph = None  # results into d_out (device-resident consumers)
+        self.graph_host = None  # results straight into the pinned h_out (requests: no copy back)
         self.bytes = nbytes + self.nout * 8
         self.atoms_off = off
         # the batch whose table the device workspace holds for resident use (None: any other state)
         self.resident_key = None
 
-    def launch_call(self):
+    def launch_call(self, out=None):
+        """The evaluation's launches; results into out (d_out by default; the pinned h_out is
+        written directly by the last kernel, which saves the copy back and its wait)."""
         eng = self.eng
         rc = L.load().tff_serve_energy_forces(eng.cd, eng.coul, eng.order, L.ptr(eng.moduli.table), *eng.cap_args,
-                                              L.ptr(self.d_out), L.ptr(self.ws), self.ws.numel(), self.sptr)
+                                              L.ptr(self.d_out if out is None else out), L.ptr(self.ws),
+                                              self.ws.numel(), self.sptr)
         L.check(rc, "serve energy+forces")
 
     def upload(self):
@@ -197,13 +204,18 @@ class _Slot:
         return True
 
     def launch(self):
-        """Uploads the batch, runs it (graph replay when captured) and queues the copy back."""
-        n = 5 * self.eng.caps["max_systems"] + 3 * self.natoms
+        """Uploads the batch and runs it (graph replay when captured) with the results stored into
+        the pinned h_out by the last kernel; the event marks their arrival.  Measured against
+        results in device memory copied back (same box, 4096 requests of 20-1000 waters): p50 0.095
+        against 0.109 ms at 2000 req/s, 115k against 79-105k evals/s at saturation (the copy's
+        wait after a graph replay is 25-50 us)."""
         with torch.cuda.stream(self.stream):
             # the upload stays outside the graph: its size follows the batch
             self.upload()
-            self.run()
-            self.h_out[:n].copy_(self.d_out[:n], non_blocking=True)
+            if self.graph_host is not None:
+                self.graph_host.replay()
+            else:
+                self.launch_call(self.h_out)
             self.event.record(self.stream)
 
     def finish(self, now):
@@ -265,6 +277,10 @@ class ServingEngine:
             with torch.cuda.graph(g, stream=s.stream):
                 s.launch_call()
             s.graph = g
+            gh = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gh, stream=s.stream):
+                s.launch_call(s.h_out)
+            s.graph_host = gh
         torch.cuda.synchronize(self.device)
 
     def fits_alone(self, r):
