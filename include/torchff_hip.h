@@ -352,8 +352,10 @@ int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void
  * a head, one record per system (tff_serve_table_add), and from byte tff_serve_atoms_offset on
  * the atom records, 8 float64 each: x, y, z (nm), charge (e), sigma (nm), epsilon (kJ/mol),
  * molecule id (equal within a molecule: the exclusion group), 0.  tff_serve_upload copies the
- * head, the records and the atom records up to the highest one used into the workspace, with one
- * copy; tff_serve_energy_forces evaluates it into out (float64, device memory or pinned host
+ * head, the records and the atom records up to the highest one used into the workspace: a
+ * 16-byte aligned table is read by one kernel, which takes the size from the head when it runs
+ * (so the upload can be captured into one graph with the evaluation and serve batches of any
+ * size), another by one copy; tff_serve_energy_forces evaluates it into out (float64, device memory or pinned host
  * memory mapped for the device, written with plain stores only): 5 values per system in
  * table order [real space (LJ + Coulomb incl. the exclusion correction), 0, reciprocal, self,
  * total] at out[5 s], then from out[5 max_systems] the forces -dE/dr (3 per atom record, rows of

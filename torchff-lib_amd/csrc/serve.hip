@@ -41,6 +41,7 @@ constexpr double SQ_SCALE = 1099511627776.0;  // 2^40: charge grids
 constexpr int SERVE_LINES = 16;               // lines per FFT tile
 constexpr int FFT_BLOCKS = 512;               // workgroups of an FFT pass (each loops over tiles)
 constexpr int REC = 8;                        // float64 values per atom record
+constexpr int STAGE_BLOCKS = 128;             // workgroups of the table staging kernel
 constexpr int RW = 8;                         // waves per workgroup
 constexpr int MAX_SPLITS = 32;                // real-space workgroups per 64-atom task, at most
 constexpr int UNITS_PER_WAVE = 1;             // target (chunk x image) units per wave
@@ -551,6 +552,23 @@ __global__ __launch_bounds__(RW * TFF_WAVE) void k_serve_gather(const BatchHead*
   }
 }
 
+// The batch table (head, system records, atom records up to the highest one used) copied from
+// pinned host memory into the workspace.  The size is read from the table's head when the kernel
+// runs, so a captured graph that holds this launch serves batches of any size.  Sizes are
+// multiples of 16 bytes (the records start 256-byte aligned, atom records are 64 bytes).
+__global__ __launch_bounds__(256) void k_serve_stage(const uint4* __restrict__ table, uint4* __restrict__ input,
+                                                     long long rec_bytes, int max_atoms) {
+  __shared__ long long s_n;
+  if (threadIdx.x == 0) {
+    int ae = ((const BatchHead*)table)->atom_end;
+    ae = ae < 0 ? 0 : (ae > max_atoms ? max_atoms : ae);
+    s_n = (rec_bytes + (long long)REC * 8 * ae) / 16;
+  }
+  __syncthreads();
+  const long long n = s_n;
+  for (long long k = blockIdx.x * 256LL + threadIdx.x; k < n; k += (long long)gridDim.x * 256) input[k] = table[k];
+}
+
 __global__ void k_serve_zero(unsigned long long* __restrict__ p, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = 0ull;
@@ -845,11 +863,19 @@ extern "C" int tff_serve_upload(const void* table_host, int compute_dtype, int m
                   max_systems, max_atoms);
     return TFF_ERR_ARG;
   }
-  const size_t bytes = atoms_offset(max_systems) + (size_t)REC * 8 * h->atom_end;
-  if (hipMemcpyAsync(w.input, table_host, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
-    tff_set_error("serve: table upload failed");
-    return TFF_ERR_LAUNCH;
+  if ((uintptr_t)table_host % 16 != 0) {
+    const size_t bytes = atoms_offset(max_systems) + (size_t)REC * 8 * h->atom_end;
+    if (hipMemcpyAsync(w.input, table_host, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) {
+      tff_set_error("serve: table upload failed");
+      return TFF_ERR_LAUNCH;
+    }
+    return TFF_OK;
   }
+  // a kernel reads the pinned table directly (no copy command; the size comes from the head at
+  // run time, so the launch can be captured together with the evaluation)
+  hipLaunchKernelGGL(k_serve_stage, dim3(STAGE_BLOCKS), dim3(256), 0, stream, (const uint4*)table_host,
+                     (uint4*)w.input, (long long)atoms_offset(max_systems), max_atoms);
+  TFF_CHECK_LAUNCH("serve upload");
   return TFF_OK;
 }
 

@@ -9,10 +9,11 @@ into a HIP graph and replayed for every batch:
 * a **slot** owns a stream, a device workspace sized by fixed capacities
   (systems, 64-atom tasks, FFT tiles per pass, largest grid edge, grid points,
   atoms), pinned host staging for inputs / outputs, the host table of the
-  batch, and two captured graphs: one whose last kernel stores the results
-  straight into the pinned output block (requests: no copy back, whose wait
-  after a replay measured 25-50 us), one writing them to device memory
-  (device-resident consumers such as the MD model);
+  batch, and two captured graphs: one that uploads the batch (a kernel reading
+  the pinned table, sized by the table's head at run time), evaluates it and
+  stores the results straight into the pinned output block (requests: no copy
+  command in either direction), and one that evaluates the uploaded batch into
+  device memory (device-resident consumers such as the MD model);
 * **admission by HBM footprint**: a request joins the batch being formed only
   if its atoms, grid points, tasks and tiles fit what is left of the slot's
   capacities (``tff_serve_table_add`` refuses it otherwise); the capacities
@@ -204,17 +205,18 @@ class _Slot:
         return True
 
     def launch(self):
-        """Uploads the batch and runs it (graph replay when captured) with the results stored into
-        the pinned h_out by the last kernel; the event marks their arrival.  Measured against
-        results in device memory copied back (same box, 4096 requests of 20-1000 waters): p50 0.095
-        against 0.109 ms at 2000 req/s, 115k against 79-105k evals/s at saturation (the copy's
-        wait after a graph replay is 25-50 us)."""
+        """Uploads the batch and runs it (one graph replay when captured: the upload kernel reads
+        the batch size from the pinned table when it runs) with the results stored into the pinned
+        h_out by the last kernel; the event marks their arrival.  Measured against results in
+        device memory copied back (same box, 4096 requests of 20-1000 waters): p50 0.095 against
+        0.109 ms at 2000 req/s, 115k against 79-105k evals/s at saturation (the copy's wait after a
+        graph replay is 25-50 us)."""
+        self.resident_key = None
         with torch.cuda.stream(self.stream):
-            # the upload stays outside the graph: its size follows the batch
-            self.upload()
             if self.graph_host is not None:
                 self.graph_host.replay()
             else:
+                self.upload()
                 self.launch_call(self.h_out)
             self.event.record(self.stream)
 
@@ -279,6 +281,7 @@ class ServingEngine:
             s.graph = g
             gh = torch.cuda.CUDAGraph()
             with torch.cuda.graph(gh, stream=s.stream):
+                s.upload()
                 s.launch_call(s.h_out)
             s.graph_host = gh
         torch.cuda.synchronize(self.device)
