@@ -51,11 +51,11 @@ int tff_prof_collect(const char* name, double* total_ms, long long* count);
  * search and pair kernel; 3: the whole chain on the auxiliary stream.  Other values mean 1. */
 void tff_fused_set_overlap(int mode);
 /* Graph replay of tff_fused_energy_forces (no counterpart in the reference; results are identical
- * either way).  The launches of a call up to its last kernel are captured once into a HIP graph
- * per device, input and workspace addresses, box and parameters (up to 8 kept per device, the
- * least recently used dropped after a device synchronisation), and replayed by later calls with
- * the same key; the last kernel, which writes forces_out, energies_out and the status words, is
- * launched per call.  mode 1 (default): for float32 arithmetic below 300 000 atoms (where replay
+ * either way).  The launches of a call up to its last kernel are captured into a HIP graph per
+ * device, input and workspace addresses, box and parameters when two calls in a row on a device
+ * have the same such key (up to 8 graphs kept per device, the least recently used dropped after a
+ * device synchronisation), and replayed by later calls with that key; the last kernel, which
+ * writes forces_out, energies_out and the status words, is launched per call.  mode 1 (default): for float32 arithmetic below 300 000 atoms (where replay
  * measured faster); 2: always; 0: never.  A call whose stream is being captured, or made while
  * tff_prof_enable(1) is on, launches its kernels one by one.  Other values mean 1. */
 void tff_fused_set_graphs(int mode);

@@ -2552,6 +2552,8 @@ std::mutex g_graph_mu;
 std::vector<GraphEntry> g_graphs[64];
 unsigned long long g_graph_clock = 0;
 hipStream_t g_capture_stream[64];
+FusedKey g_seen[64];  // per device: the key of the last call that found no graph
+bool g_seen_valid[64];
 
 // a graph is captured and launched for a call on st: by the mode (0: never; 1: float32 arithmetic
 // below SPREAD_FIRST_ATOMS atoms; 2: always), no event timing requested, st not itself being
@@ -2569,20 +2571,19 @@ bool graphs_usable(hipStream_t st, bool f32, long long n) {
   return cs == hipStreamCaptureStatusNone;
 }
 
-// The executable graph of key: found in the cache, or captured by launch_core(capture stream) and
-// instantiated.  *ex stays null when this call should run without a graph (capture unavailable);
-// a failure of launch_core itself is returned.
+// Captures key's forward core into a new cache entry (caller holds g_graph_mu) when the previous
+// uncached key seen on this device was the same key: a key seen once (inputs at new addresses on
+// every call, one-off calls) is not captured.  *ex stays null when the call should run without a
+// graph; a failure of launch_core itself is returned.
 template <typename F>
-int graph_for(const FusedKey& key, F&& launch_core, hipGraphExec_t* ex) {
+int graph_capture(const FusedKey& key, F&& launch_core, hipGraphExec_t* ex) {
   *ex = nullptr;
-  std::lock_guard<std::mutex> lk(g_graph_mu);
+  if (!g_seen_valid[key.dev] || memcmp(&g_seen[key.dev], &key, sizeof key) != 0) {
+    g_seen[key.dev] = key;
+    g_seen_valid[key.dev] = true;
+    return TFF_OK;
+  }
   std::vector<GraphEntry>& cache = g_graphs[key.dev];
-  for (GraphEntry& e : cache)
-    if (memcmp(&e.key, &key, sizeof key) == 0) {
-      e.used = ++g_graph_clock;
-      *ex = e.ex;
-      return TFF_OK;
-    }
   hipStream_t& cs = g_capture_stream[key.dev];
   if (!cs && hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
     cs = nullptr;
@@ -2623,7 +2624,34 @@ int graph_for(const FusedKey& key, F&& launch_core, hipGraphExec_t* ex) {
     cache.erase(cache.begin() + (long)v);
   }
   cache.push_back({key, g, ++g_graph_clock});
+  g_seen_valid[key.dev] = false;
   *ex = g;
+  return TFF_OK;
+}
+
+// Launches the executable graph of key on st, from the cache or newly captured (graph_capture).
+// The launch happens under the cache's lock, so no other thread evicts the graph meanwhile.
+// *launched stays false when this call should run without a graph.
+template <typename F>
+int graph_launch(const FusedKey& key, F&& launch_core, hipStream_t st, bool* launched) {
+  *launched = false;
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  hipGraphExec_t ex = nullptr;
+  for (GraphEntry& e : g_graphs[key.dev])
+    if (memcmp(&e.key, &key, sizeof key) == 0) {
+      e.used = ++g_graph_clock;
+      ex = e.ex;
+      break;
+    }
+  if (!ex) {
+    const int rc = graph_capture(key, launch_core, &ex);
+    if (rc || !ex) return rc;
+  }
+  if (hipGraphLaunch(ex, st) != hipSuccess) {
+    tff_set_error("fused: graph launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return TFF_ERR_LAUNCH;
+  }
+  *launched = true;
   return TFF_OK;
 }
 
@@ -2658,16 +2686,10 @@ int fused_forward_t(int io_dtype, int compute_dtype, const TI* coords, const TI*
     key.d[10] = list_cutoff;
     key.d[11] = alpha;
     key.d[12] = coul;
-    hipGraphExec_t ex = nullptr;
-    const int rc = graph_for(key, core, &ex);
+    bool launched = false;
+    const int rc = graph_launch(key, core, st, &launched);
     if (rc) return rc;
-    if (ex) {
-      if (hipGraphLaunch(ex, st) != hipSuccess) {
-        tff_set_error("fused: graph launch failed: %s", hipGetErrorString(hipGetLastError()));
-        return TFF_ERR_LAUNCH;
-      }
-      return launch_fused_finalize<TI>(w, g, coul, alpha, energies, forces, status_out, status_host, st);
-    }
+    if (launched) return launch_fused_finalize<TI>(w, g, coul, alpha, energies, forces, status_out, status_host, st);
   }
   const int rc = core(st);
   if (rc) return rc;
