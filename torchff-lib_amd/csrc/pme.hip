@@ -106,6 +106,9 @@ __device__ __forceinline__ void lattice_multipoles(const Multipoles& mp, long lo
 // point is written by its owner.
 constexpr int SB = 8;
 constexpr int SB_SRC = 4;  // source bricks per axis
+// a brick in LDS: x fastest, rows of SB + 1 and planes of (SB + 1) SB + 1 doubles, so that the
+// atomics of lanes at other x, y or z fall on different banks
+constexpr int SB_Y = SB + 1, SB_Z = SB_Y * SB + 1;
 
 __device__ __forceinline__ void atom_base(const double* r, const PmeGeom& g, int a, int P, int& base, double& t) {
   const double s = (r[0] * g.inv[a] + r[1] * g.inv[3 + a] + r[2] * g.inv[6 + a]) * (double)g.K[a];
@@ -202,7 +205,7 @@ template <typename T, int P, int LVL>
 __global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Multipoles mp, PmeGeom g,
                                                 const int* __restrict__ start, const int* __restrict__ list,
                                                 double* __restrict__ grid) {
-  __shared__ double s_b[SB * SB * SB];
+  __shared__ double s_b[SB_Z * SB];
   const int K0 = g.K[0], K1 = g.K[1], K2 = g.K[2];
   const int nb[3] = {(K0 + SB - 1) / SB, (K1 + SB - 1) / SB, (K2 + SB - 1) / SB};
   const int bid = blockIdx.x;
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Mu
     B0[a] = B[a] * SB;
     Bn[a] = g.K[a] - B0[a] < SB ? g.K[a] - B0[a] : SB;
   }
-  for (int k = threadIdx.x; k < SB * SB * SB; k += blockDim.x) s_b[k] = 0.0;
+  for (int k = threadIdx.x; k < SB_Z * SB; k += blockDim.x) s_b[k] = 0.0;
   // source bricks: per axis those holding a stencil base in [B0 - P + 1, B0 + Bn - 1] (periodic);
   // their atom ranges are concatenated so that every thread takes atoms of the whole set
   __shared__ int s_a0[SB_SRC * SB_SRC * SB_SRC], s_pref[SB_SRC * SB_SRC * SB_SRC + 1], s_nc;
@@ -277,10 +280,10 @@ __global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Mu
               for (int kb = 0; kb < P; ++kb) {
                 if (loc[1][kb] < 0) continue;
                 const T wab = wa * Mv[1][kb];
-                double* row = s_b + (loc[0][ka] * SB + loc[1][kb]) * SB;
+                double* row = s_b + loc[1][kb] * SB_Y + loc[0][ka];
 #pragma unroll
                 for (int kc = 0; kc < P; ++kc)
-                  if (loc[2][kc] >= 0) atomicAdd(&row[loc[2][kc]], (double)(wab * Mv[2][kc]));
+                  if (loc[2][kc] >= 0) atomicAdd(&row[loc[2][kc] * SB_Z], (double)(wab * Mv[2][kc]));
               }
             }
           } else {
@@ -304,13 +307,13 @@ __global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Mu
                   c1 += tc[4] * dxy + tc[5] * xdy;
                   c2 = tc[2] * xy;
                 }
-                double* row = s_b + (loc[0][ka] * SB + loc[1][kb]) * SB;
+                double* row = s_b + loc[1][kb] * SB_Y + loc[0][ka];
 #pragma unroll
                 for (int kc = 0; kc < P; ++kc) {
                   if (loc[2][kc] < 0) continue;
                   T v = c0 * Mv[2][kc] + c1 * dM[2][kc];
                   if constexpr (LVL >= 2) v += c2 * d2M[2][kc];
-                  atomicAdd(&row[loc[2][kc]], (double)v);
+                  atomicAdd(&row[loc[2][kc] * SB_Z], (double)v);
                 }
               }
             }
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(256) void k_spread(const T* __restrict__ coords, Mu
   for (int k = threadIdx.x; k < SB * SB * SB; k += blockDim.x) {
     const int lz = k % SB, ly = (k / SB) % SB, lx = k / (SB * SB);
     if (lx < Bn[0] && ly < Bn[1] && lz < Bn[2])
-      grid[((long long)(B0[0] + lx) * K1 + (B0[1] + ly)) * K2 + (B0[2] + lz)] = s_b[k];
+      grid[((long long)(B0[0] + lx) * K1 + (B0[1] + ly)) * K2 + (B0[2] + lz)] = s_b[lz * SB_Z + ly * SB_Y + lx];
   }
 }
 
