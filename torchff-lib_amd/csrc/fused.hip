@@ -2251,24 +2251,41 @@ int run_fused_core(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
   hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbB), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
                      w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
                      w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
-  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
-                     col, w.excl_part, status);
-  // 3. PME on the auxiliary stream (needs only the cluster arrays), concurrent with 4-5
+  // 3. PME on the auxiliary stream (needs only the cluster arrays), concurrent with 4-5; the
+  // exclusion table (needed from the pair search on) beside the first reciprocal-space kernels
   PmeGeom pg = make_geom(box, K, order, alpha);
   FftPlan pl[3];
   for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
   const int mode = g_overlap.load();
   const bool spread_first = mode == 2 || (mode == 1 && n >= SPREAD_FIRST_ATOMS);
+  auto fill_exclusions = [&](hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, s, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
+                       col, w.excl_part, status);
+  };
+  auto spread = [&](hipStream_t s) {
+    void* tok = tff_prof_begin("fused_spread", s);
+    const int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, nullptr, coul, true, status, s);
+    tff_prof_end(tok, s);
+    return rc;
+  };
   hipStream_t sp = st;
-  for (int pass = 0; pass < 2; ++pass) {
-    if (pass == 1 && !aux_stream_fork(st, &sp)) {
+  if (!aux_stream_fork(st, &sp)) {
+    tff_set_error("fused: could not create the auxiliary stream");
+    return TFF_ERR_LAUNCH;
+  }
+  if (spread_first) {
+    // the spread alone on the caller's stream, the exclusion table beside it; the FFT and the
+    // gather fork after the spread
+    fill_exclusions(sp);
+    const int rc = spread(st);
+    if (rc) return rc;
+    aux_stream_join(st, sp);
+    if (!aux_stream_fork(st, &sp)) {
       tff_set_error("fused: could not create the auxiliary stream");
       return TFF_ERR_LAUNCH;
     }
-    if (spread_first != (pass == 0)) continue;
-    void* tok = tff_prof_begin("fused_spread", sp);
-    int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, nullptr, coul, true, status, sp);
-    tff_prof_end(tok, sp);
+  } else {
+    const int rc = spread(sp);
     if (rc) return rc;
   }
   TC* phi = (TC*)w.grid;  // the real potential grid reuses the charge grid's memory after the z pass
@@ -2279,6 +2296,7 @@ int run_fused_core(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
     tff_prof_end(tok, sp);
     if (rc) return rc;
   }
+  if (!spread_first) fill_exclusions(st);
   // 4. cluster pairs
   {
     void* tok = tff_prof_begin("fused_cluster_pairs", st);
