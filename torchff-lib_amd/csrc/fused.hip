@@ -605,7 +605,9 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ nc
 // Segments -> one compact cluster-pair list in (ci, segment order) (deterministic), with the
 // sub-tile masks: bit gi * 4 + gj is set when the bounding boxes of group gi of ci and group
 // gj of cj are within cut2 (minimum image of the group centres: a lower bound on the distance
-// of every image of every atom pair of the two groups).
+// of every image of every atom pair of the two groups).  One wave per i-cluster, one lane per
+// tile (64 tiles per step): a lane loads its tile's entry, then the four group boxes of cj, and
+// forms all 16 bits itself (ci's group boxes are the same for every lane).
 __global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ ncl_ptr, const int2* __restrict__ seg,
                                                       const int* __restrict__ seg_count,
                                                       const int* __restrict__ seg_start, int maxn,
@@ -615,36 +617,36 @@ __global__ __launch_bounds__(FB) void k_compact_pairs(const int* __restrict__ nc
   const int ci = (blockIdx.x * FB + threadIdx.x) / TFF_WAVE;
   if (ci >= *ncl_ptr) return;
   const int n = seg_count[ci], o = seg_start[ci];
-  // 16 tiles per step: lane = (tile lane >> 2, i-group lane & 3), looping over the j-groups
-  const int gi = lane & 3, sub = lane >> 2;
-  const float4 c0 = sb_c[ci * 4 + gi], h0 = sb_h[ci * 4 + gi];
+  float4 c0[4], h0[4];
+#pragma unroll
+  for (int gi = 0; gi < 4; ++gi) {
+    c0[gi] = sb_c[ci * 4 + gi];
+    h0[gi] = sb_h[ci * 4 + gi];
+  }
   const float L0 = (float)g.L[0], L1 = (float)g.L[1], L2 = (float)g.L[2];
   const float i0 = (float)g.invL[0], i1 = (float)g.invL[1], i2 = (float)g.invL[2];
-  for (int k0 = 0; k0 < n; k0 += 16) {
-    const int k = k0 + sub;
+  for (int k = lane; k < n; k += TFF_WAVE) {
+    const int2 e = seg[(long long)ci * maxn + k];
+    const int cj = e.x;
     unsigned bits = 0;
-    int2 e = make_int2(0, 0);
-    if (k < n) {
-      e = seg[(long long)ci * maxn + k];
-      const int cj = e.x;
 #pragma unroll
-      for (int gj = 0; gj < 4; ++gj) {
-        const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
-        if (h0.x >= 0.f && h1.x >= 0.f && (ci != cj || gj >= gi)) {
-          float dx = c0.x - c1.x, dy = c0.y - c1.y, dz = c0.z - c1.z;
+    for (int gj = 0; gj < 4; ++gj) {
+      const float4 c1 = sb_c[cj * 4 + gj], h1 = sb_h[cj * 4 + gj];
+#pragma unroll
+      for (int gi = 0; gi < 4; ++gi) {
+        if (h0[gi].x >= 0.f && h1.x >= 0.f && (ci != cj || gj >= gi)) {
+          float dx = c0[gi].x - c1.x, dy = c0[gi].y - c1.y, dz = c0[gi].z - c1.z;
           dx -= L0 * rintf(dx * i0);
           dy -= L1 * rintf(dy * i1);
           dz -= L2 * rintf(dz * i2);
-          const float gx = fmaxf(fabsf(dx) - h0.x - h1.x, 0.f);
-          const float gy = fmaxf(fabsf(dy) - h0.y - h1.y, 0.f);
-          const float gz = fmaxf(fabsf(dz) - h0.z - h1.z, 0.f);
+          const float gx = fmaxf(fabsf(dx) - h0[gi].x - h1.x, 0.f);
+          const float gy = fmaxf(fabsf(dy) - h0[gi].y - h1.y, 0.f);
+          const float gz = fmaxf(fabsf(dz) - h0[gi].z - h1.z, 0.f);
           if (gx * gx + gy * gy + gz * gz <= cut2) bits |= 1u << (gi * 4 + gj);
         }
       }
     }
-    bits |= __shfl_xor(bits, 1, TFF_WAVE);
-    bits |= __shfl_xor(bits, 2, TFF_WAVE);
-    if (k < n && gi == 0) pairs[o + k] = make_int4(ci, e.x, (int)((unsigned)e.y | bits), 0);
+    pairs[o + k] = make_int4(ci, cj, (int)((unsigned)e.y | bits), 0);
   }
 }
 
