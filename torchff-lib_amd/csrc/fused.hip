@@ -482,7 +482,7 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
                                                       int2* __restrict__ seg, int* __restrict__ seg_count,
                                                       int* __restrict__ status) {
   __shared__ int s_xp[FB / TFF_WAVE][CL * MAXEX];
-  __shared__ int s_off[FB / TFF_WAVE][TFF_WAVE], s_cs[FB / TFF_WAVE][TFF_WAVE];
+  __shared__ int s_off[FB / TFF_WAVE][2 * TFF_WAVE], s_cs[FB / TFF_WAVE][2 * TFF_WAVE];
   const int lane = threadIdx.x & (TFF_WAVE - 1);
   const int w = threadIdx.x / TFF_WAVE;
   const int ci = blockIdx.x * (FB / TFF_WAVE) + w;
@@ -513,21 +513,73 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
   const int x0 = nx == g.ncol[0] ? 0 : cx - g.reach[0];
   const int y0 = ny == g.ncol[1] ? 0 : cy - g.reach[1];
   const int ncc = nx * ny;
-  // Candidates: the neighbouring columns (up to 64 at a time, one per lane) clipped to cj >= ci,
-  // scanned into one flat range that the wave walks 64 clusters at a time (column-major order).
+  // The z window of ci: clusters whose z extent meets [z_lo - rlist, z_hi + rlist] (or its
+  // periodic images) can pass the bounding-box test below; every other one fails it.  The
+  // clusters of a column are z ordered (their lower and upper z bounds rise along the column), so
+  // per column the window is one index range, or two when it wraps round the box; 0.01 nm of slack
+  // keeps rounding from dropping a cluster the test would accept.  The accepted clusters and their
+  // order (columns in order, clusters ascending) are those of a scan over whole columns.
+  const float rl = sqrtf(rlist2) + 0.01f;
+  const double zw_lo = (double)(c0v.z - h0.z) - rl, zw_hi = (double)(c0v.z + h0.z) + rl;
+  const bool zall = zw_hi - zw_lo >= g.L[2];
+  const float za0 = zw_lo < 0 ? 0.f : (float)zw_lo, zb0 = zw_hi > g.L[2] ? (float)g.L[2] : (float)zw_hi;
+  const bool zwrap = !zall && (zw_lo < 0 || zw_hi > g.L[2]);
+  const float za1 = zw_lo < 0 ? (float)(zw_lo + g.L[2]) : 0.f;  // second interval [za1, zb1]
+  const float zb1 = zw_lo < 0 ? (float)g.L[2] : (float)(zw_hi - g.L[2]);
+  // Candidates: the neighbouring columns (up to 64 at a time, one per lane), each as up to two
+  // cluster ranges inside the window, clipped to cj >= ci, scanned into one flat range that the
+  // wave walks 64 clusters at a time (column-major order).
   for (int c0 = 0; c0 < ncc; c0 += TFF_WAVE) {
     const int cc = c0 + lane;
-    int cs = 0, cnt = 0;
+    int r0s = 0, r0n = 0, r1s = 0, r1n = 0;
     if (cc < ncc) {
       int xx = x0 + cc / ny, yy = y0 + cc % ny;
       xx += xx < 0 ? g.ncol[0] : (xx >= g.ncol[0] ? -g.ncol[0] : 0);
       yy += yy < 0 ? g.ncol[1] : (yy >= g.ncol[1] ? -g.ncol[1] : 0);
       const int col_id = xx * g.ncol[1] + yy;
       const int ce = col_cl_start[col_id + 1];
-      cs = col_cl_start[col_id];
-      cs = cs > ci ? cs : ci;
-      cnt = ce > cs ? ce - cs : 0;
+      const int cs = col_cl_start[col_id];
+      // first cluster with z max >= a, last cluster with z min <= b (both monotone in the column)
+      auto first_ge = [&](float a) {
+        int lo = cs, hi = ce;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (bb_c[mid].z + bb_h[mid].z >= a) hi = mid; else lo = mid + 1;
+        }
+        return lo;
+      };
+      auto last_le = [&](float b) {
+        int lo = cs - 1, hi = ce - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (bb_c[mid].z - bb_h[mid].z <= b) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+      };
+      int a0 = cs, e0 = ce, a1 = 0, e1 = 0;  // [start, end) of up to two ranges
+      if (!zall && cs < ce) {
+        if (!zwrap) {
+          a0 = first_ge(za0);
+          e0 = last_le(zb0) + 1;
+        } else {
+          // [0, b] at the column's start, [a, L) at its end; merged when they meet
+          const int l0 = last_le(zw_lo < 0 ? zb0 : zb1) + 1;
+          const int f1 = first_ge(zw_lo < 0 ? za1 : za0);
+          if (f1 > l0) {
+            e0 = l0;
+            a1 = f1;
+            e1 = ce;
+          }
+        }
+      }
+      a0 = a0 > ci ? a0 : ci;
+      a1 = a1 > ci ? a1 : ci;
+      r0s = a0;
+      r0n = e0 > a0 ? e0 - a0 : 0;
+      r1s = a1;
+      r1n = e1 > a1 ? e1 - a1 : 0;
     }
+    const int cnt = r0n + r1n;
     int incl = cnt;
 #pragma unroll
     for (int off = 1; off < TFF_WAVE; off <<= 1) {
@@ -535,18 +587,21 @@ __global__ __launch_bounds__(FB) void k_cluster_pairs(const int* __restrict__ nc
       if (lane >= off) incl += v;
     }
     const int total = __shfl(incl, TFF_WAVE - 1, TFF_WAVE);
-    s_off[w][lane] = incl - cnt;
-    s_cs[w][lane] = cs;
+    s_off[w][2 * lane] = incl - cnt;
+    s_cs[w][2 * lane] = r0s;
+    s_off[w][2 * lane + 1] = incl - cnt + r0n;
+    s_cs[w][2 * lane + 1] = r1s;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int nvalid = ncc - c0 < TFF_WAVE ? ncc - c0 : TFF_WAVE;
+    const int nvalid = 2 * (ncc - c0 < TFF_WAVE ? ncc - c0 : TFF_WAVE);
     for (int b = 0; b < total; b += TFF_WAVE) {
       const int k = b + lane;
       unsigned info = 0;
       int cj = -1;
       if (k < total) {
-        // last column whose offset is <= k (offsets are non-decreasing)
+        // last range whose offset is <= k (offsets are non-decreasing; an empty range shares its
+        // offset with the next one, which then holds k)
         int lo = 0, hi = nvalid - 1;
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
