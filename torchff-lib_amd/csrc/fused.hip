@@ -2448,6 +2448,46 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
                      w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
                      w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
   hipLaunchKernelGGL((k_fill_vslots<TI, TC>), dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, gF, (V4<TC>*)w.vsl);
+  // the reciprocal-space part (charge and dipole grids, both transforms, the gather) needs only
+  // the cluster arrays and the per-slot upstream forces: it runs on the auxiliary stream beside the
+  // pair search and the pair kernel, joined before the final kernel
+  hipStream_t sp = st;
+  if (!aux_stream_fork(st, &sp)) {
+    tff_set_error("fused backward: could not create the auxiliary stream");
+    return TFF_ERR_LAUNCH;
+  }
+  PmeGeom pg = make_geom(box, K, order, alpha);
+  FftPlan pl[3];
+  for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
+  const unsigned nbricks = (unsigned)(((K[0] + BR - 1) / BR) * ((K[1] + BR - 1) / BR) * ((K[2] + BR - 1) / BR));
+  V4<TC> jsc{(TC)(K[0] * box.inv[0]), (TC)(K[1] * box.inv[4]), (TC)(K[2] * box.inv[8]), (TC)0};
+  TC* phi = (TC*)w.grid;
+  TC* psi = (TC*)w.grid_d;
+  const unsigned nb = (unsigned)((nslots + FB - 1) / FB);
+#define TFF_GRAD_PME(P)                                                                                          \
+  {                                                                                                              \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P, false>), dim3(nbricks), dim3(256), 0, sp, w.ncl,                  \
+                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status, \
+                       (const V4<TC>*)nullptr, V4<TC>{});                                                        \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P, true>), dim3(nbricks), dim3(256), 0, sp, w.ncl,                   \
+                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid_d,       \
+                       status, (const V4<TC>*)w.vsl, jsc);                                                       \
+    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts,          \
+                      "fused_grad_fft", sp);                                                                     \
+    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid_d, 1.0 / Q_SCALE, psi, w.kparts,                 \
+                      "fused_grad_fft", sp);                                                                     \
+    hipLaunchKernelGGL((k_gather_grad<TC, P>), dim3(nb), dim3(FB), 0, sp, w.ncl, phi, psi, (const V4<TC>*)w.xyzq, \
+                       (const V4<TC>*)w.vsl, w.slot_orig, pg, coul, gE, w.gpme);                                 \
+  }
+  switch (order) {
+    case 3: TFF_GRAD_PME(3); break;
+    case 4: TFF_GRAD_PME(4); break;
+    case 5: TFF_GRAD_PME(5); break;
+    case 6: TFF_GRAD_PME(6); break;
+    case 7: TFF_GRAD_PME(7); break;
+    default: TFF_GRAD_PME(8); break;
+  }
+#undef TFF_GRAD_PME
   hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
                      col, w.excl_part, status);
   hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c, w.bb_h,
@@ -2471,38 +2511,7 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
                        gE, w.gacc);
     tff_prof_end(tok, st);
   }
-  PmeGeom pg = make_geom(box, K, order, alpha);
-  FftPlan pl[3];
-  for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
-  const unsigned nbricks = (unsigned)(((K[0] + BR - 1) / BR) * ((K[1] + BR - 1) / BR) * ((K[2] + BR - 1) / BR));
-  V4<TC> jsc{(TC)(K[0] * box.inv[0]), (TC)(K[1] * box.inv[4]), (TC)(K[2] * box.inv[8]), (TC)0};
-  TC* phi = (TC*)w.grid;
-  TC* psi = (TC*)w.grid_d;
-  const unsigned nb = (unsigned)((nslots + FB - 1) / FB);
-#define TFF_GRAD_PME(P)                                                                                          \
-  {                                                                                                              \
-    hipLaunchKernelGGL((k_spread_bricks<TC, P, false>), dim3(nbricks), dim3(256), 0, st, w.ncl,                  \
-                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status, \
-                       (const V4<TC>*)nullptr, V4<TC>{});                                                        \
-    hipLaunchKernelGGL((k_spread_bricks<TC, P, true>), dim3(nbricks), dim3(256), 0, st, w.ncl,                   \
-                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid_d,       \
-                       status, (const V4<TC>*)w.vsl, jsc);                                                       \
-    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts,          \
-                      "fused_grad_fft", st);                                                                     \
-    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid_d, 1.0 / Q_SCALE, psi, w.kparts,                 \
-                      "fused_grad_fft", st);                                                                     \
-    hipLaunchKernelGGL((k_gather_grad<TC, P>), dim3(nb), dim3(FB), 0, st, w.ncl, phi, psi, (const V4<TC>*)w.xyzq, \
-                       (const V4<TC>*)w.vsl, w.slot_orig, pg, coul, gE, w.gpme);                                 \
-  }
-  switch (order) {
-    case 3: TFF_GRAD_PME(3); break;
-    case 4: TFF_GRAD_PME(4); break;
-    case 5: TFF_GRAD_PME(5); break;
-    case 6: TFF_GRAD_PME(6); break;
-    case 7: TFF_GRAD_PME(7); break;
-    default: TFF_GRAD_PME(8); break;
-  }
-#undef TFF_GRAD_PME
+  aux_stream_join(st, sp);
   hipLaunchKernelGGL(k_finalize_grad<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.gacc, w.gpme, q, coul,
                      alpha, gE, dr, dq, dsig, deps, status, status_host);
   TFF_CHECK_LAUNCH("fused energy+forces backward");
