@@ -315,9 +315,10 @@ int tff_fused_energy_forces_noalloc(int io_dtype, int compute_dtype, const void*
  * its pure-PyTorch path by double backward (torchff/multipoles.py:244-341, torchff/vdw.py:86-111,
  * torchff/pme_ref.py:11-469 with create_graph; its CUDA ops' backward only scales stored first
  * derivatives, e.g. csrc/nonbonded/nonbonded_atom_pairs_cuda.cu:143-260).  The cluster pair list
- * is rebuilt, not stored from the forward call.  dL/depsilon_i is 0 for pairs with
- * eps_i eps_j = 0 (as the forward's first derivatives).  Same arguments and constraints as
- * tff_fused_energy_forces otherwise; workspace sized by tff_fused_backward_workspace_bytes. */
+ * is rebuilt here (tff_fused_energy_forces_backward_reuse takes the forward call's).
+ * dL/depsilon_i is 0 for pairs with eps_i eps_j = 0 (as the forward's first derivatives).  Same
+ * arguments and constraints as tff_fused_energy_forces otherwise; workspace sized by
+ * tff_fused_backward_workspace_bytes. */
 size_t tff_fused_backward_workspace_bytes(int compute_dtype, long long natoms, const double* box_host,
                                           double list_cutoff, int K0, int K1, int K2);
 int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void* coords, const void* charges,
@@ -330,6 +331,25 @@ int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void
                                      void* grad_charges, void* grad_sigma, void* grad_epsilon,
                                      int32_t* status_out, const void* box_device, int32_t* status_host,
                                      void* workspace, size_t workspace_bytes, tff_stream_t stream);
+/* As tff_fused_energy_forces_backward, but the cluster arrays, the exclusion table and the pair
+ * list are taken from the workspace as the last tff_fused_energy_forces call on it left them,
+ * instead of being rebuilt (about 80 us of a 96 000-atom water box's backward).  The caller
+ * guarantees that this call was made with the same (backward-sized) workspace, the same inputs
+ * (coordinates, charges, sigma, epsilon, box, exclusions, cutoffs, grid) and on the same stream, and
+ * that no other call has written the workspace since; the forward call's status words (capacity
+ * flags, tile count) start this call's.  No counterpart in the reference, whose autograd recomputes
+ * the forward's graph; it serves autograd callers that keep the forward's workspace. */
+int tff_fused_energy_forces_backward_reuse(int io_dtype, int compute_dtype, const void* coords,
+                                           const void* charges, const void* sigma, const void* epsilon,
+                                           long long natoms, const double* box_host,
+                                           const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff,
+                                           double list_cutoff, double ewald_alpha, double coul_constant, int K0,
+                                           int K1, int K2, int order, const void* xmoduli, const void* ymoduli,
+                                           const void* zmoduli, const double* grad_energy,
+                                           const void* grad_forces, void* grad_coords, void* grad_charges,
+                                           void* grad_sigma, void* grad_epsilon, int32_t* status_out,
+                                           const void* box_device, int32_t* status_host, void* workspace,
+                                           size_t workspace_bytes, tff_stream_t stream);
 
 /* ----------------------------------------------------- batched serving */
 /* Energy and forces of many small orthorhombic water-like systems in one fixed sequence of

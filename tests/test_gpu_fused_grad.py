@@ -122,3 +122,79 @@ def test_gpu_gradients_match_finite_differences(case, which, idx):
     minus[which][idx] -= h
     fd = (loss(plus) - loss(minus)) / (2 * h)
     assert abs(fd - g[which][idx]) <= 1e-5 * max(1.0, abs(fd)), (which, idx, fd, g[which][idx])
+
+
+class _EntryCounter:
+    """Counts the calls of the two backward entry points (rebuild / reuse) of the loaded library."""
+
+    def __init__(self):
+        from torchff import _lib as L
+        self.lib = L.load()
+        self.n = {"tff_fused_energy_forces_backward": 0, "tff_fused_energy_forces_backward_reuse": 0}
+        self.orig = {}
+
+    def __enter__(self):
+        for name in self.n:
+            f = getattr(self.lib, name)
+            self.orig[name] = f
+
+            def counted(*a, _f=f, _name=name):
+                self.n[_name] += 1
+                return _f(*a)
+            setattr(self.lib, name, counted)
+        return self
+
+    def __exit__(self, *exc):
+        for name, f in self.orig.items():
+            setattr(self.lib, name, f)
+
+    def counts(self):
+        return self.n["tff_fused_energy_forces_backward"], self.n["tff_fused_energy_forces_backward_reuse"]
+
+
+def _fm_grads(m, w, coords, q, s, e, Tt):
+    inp = _inputs(w, q, s, e)
+    c = torch.tensor(coords, dtype=torch.float64, device="cuda", requires_grad=True)
+    inp = (c,) + inp[1:]
+    _, box, qt, st, et = inp
+    _, F, _ = m(c, box, qt, st, et)
+    return inp, F
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_backward_reuse_of_the_forward_list(case, precision):
+    # a backward right after its forward takes the forward's cluster arrays and pair list, and gives
+    # the same bits as a backward that rebuilds them; a grad-enabled forward call of other
+    # coordinates in between makes the earlier call's backward rebuild (its list was overwritten)
+    w, q, s, e, T, pairs = case
+    m = _module(w, precision)
+    Tt = torch.tensor(T, device="cuda")
+    rng = np.random.default_rng(11)
+    c1 = np.asarray(w["coords"], dtype=np.float64)
+    c2 = c1 + 0.01 * rng.standard_normal(c1.shape)
+    with _EntryCounter() as cnt:
+        # alone: reuse
+        inpA, FA = _fm_grads(m, w, c1, q, s, e, Tt)
+        torch.mean((FA - Tt) ** 2).backward()
+        gA = _grads(inpA)
+        assert cnt.counts() == (0, 1)
+        torch.cuda.synchronize()
+        assert int(m.backward_status[1].item()) == int(m.status[1].item()) > 0  # tile count carried over
+        inpB, FB_ = _fm_grads(m, w, c2, q, s, e, Tt)
+        torch.mean((FB_ - Tt) ** 2).backward()
+        gB = _grads(inpB)
+        assert cnt.counts() == (0, 2)
+        # interleaved: forward 1, forward 2, backward 1 (rebuild), backward 2 (rebuild: backward 1 wrote)
+        inp1, F1 = _fm_grads(m, w, c1, q, s, e, Tt)
+        inp2, F2 = _fm_grads(m, w, c2, q, s, e, Tt)
+        torch.mean((F1 - Tt) ** 2).backward()
+        torch.mean((F2 - Tt) ** 2).backward()
+        assert cnt.counts() == (2, 2)
+        g1, g2 = _grads(inp1), _grads(inp2)
+    m.check()
+    for k, a, b in zip(NAMES, gA, g1):
+        assert np.array_equal(a, b), k
+    for k, a, b in zip(NAMES, gB, g2):
+        assert np.array_equal(a, b), k
+    # the reused and the rebuilt results differ between the two coordinate sets (the test can fail)
+    assert not np.array_equal(gA[0], gB[0])

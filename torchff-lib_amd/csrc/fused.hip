@@ -27,6 +27,7 @@
 
 #include <atomic>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "pair_math.h"
@@ -975,61 +976,87 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_f64(const int4* __restrict__
 //            d/deps_i = (.)/(2 eps_i) (0 where eps_i = 0).
 // Per slot six int64 accumulators: d/dr (3), d/dq, d/dsigma, d/deps.
 constexpr int GN = 6;
+static_assert(4 * GN == 24, "a cluster column holds 24 backward sums per lane");
 
+// per slot: the upstream force v (x, y, z) and 1 / (2 eps) (w; 0 where eps = 0), from lj's 2 sqrt(eps);
+// with status_from set, thread 0 also starts the call's status words from those of the forward call
+// whose cluster arrays and pair list are reused
 template <typename TI, typename TC>
 __global__ __launch_bounds__(FB) void k_fill_vslots(const int* __restrict__ ncl_ptr, const int* __restrict__ slot_orig,
-                                                    const TI* __restrict__ v, V4<TC>* __restrict__ vsl) {
+                                                    const TI* __restrict__ v, const V2<TC>* __restrict__ lj,
+                                                    V4<TC>* __restrict__ vsl, const int* __restrict__ status_from,
+                                                    int* __restrict__ status) {
   const int sl = blockIdx.x * FB + threadIdx.x;
+  if (status_from && sl == 0) {
+    status[0] = status_from[0];
+    status[1] = status_from[1];
+  }
   if (sl >= *ncl_ptr * CL) return;
   const int i = slot_orig[sl];
-  vsl[sl] = (i >= 0 && v) ? V4<TC>{(TC)v[3 * i], (TC)v[3 * i + 1], (TC)v[3 * i + 2], (TC)0}
-                          : V4<TC>{(TC)0, (TC)0, (TC)0, (TC)0};
+  const TC e2 = lj[sl].y;
+  const TC ehalf_inv = e2 > (TC)0 ? (TC)2 / (e2 * e2) : (TC)0;
+  vsl[sl] = (i >= 0 && v) ? V4<TC>{(TC)v[3 * i], (TC)v[3 * i + 1], (TC)v[3 * i + 2], ehalf_inv}
+                          : V4<TC>{(TC)0, (TC)0, (TC)0, ehalf_inv};
 }
 
-// erfc(u), exp(-u^2) in the accuracy of the path (float: the fitted form above)
-template <typename TC>
-__device__ __forceinline__ void pair_grad_terms(TC r2, bool excl, bool lj_on, TC kqi, TC qi, TC qj, TC si, TC sj,
-                                                TC ei2, TC ej2, const TileArgs<TC>& ta, TC& f, TC& h, TC& phi_c,
-                                                TC& dpqi, TC& dpqj, TC& dfqi, TC& dfqj, TC& phi_lj, TC& f_lj,
-                                                TC& dps, TC& dfs) {
-  const TC c2 = (TC)1.1283791670955126;  // 2 / sqrt(pi)
-  const TC rinv = rsqrt_t(r2);
-  const TC rinv2 = rinv * rinv;
-  const TC u = ta.alpha * (r2 * rinv);
-  TC ec, em;
-  erfc_exp(u, ec, em);
-  if (excl) ec -= (TC)1;
-  const TC g = fma(c2 * u, em, ec);
-  const TC rinv3 = rinv * rinv2;
-  // Coulomb (ta.coul is folded into kqi = k q_i; q_i, q_j plain)
-  phi_c = kqi * qj * ec * rinv;
-  const TC fc = -kqi * qj * g * rinv3;
-  const TC a3 = ta.alpha * ta.alpha * ta.alpha;
-  const TC hc = kqi * qj * ((TC)3 * g * rinv3 * rinv2 + (TC)2 * c2 * a3 * em * rinv2);
-  dpqi = ta.coul * qj * ec * rinv;
-  dpqj = kqi * ec * rinv;
-  dfqi = -ta.coul * qj * g * rinv3;
-  dfqj = -kqi * g * rinv3;
-  // Lennard-Jones (sigma / 2 and 2 sqrt(eps) per atom)
-  TC eps4 = ei2 * ej2;
-  if (!lj_on) eps4 = (TC)0;
-  const TC sg = si + sj;
-  const TC s2 = sg * sg * rinv2;
-  const TC s6 = s2 * s2 * s2;
-  const TC s12 = s6 * s6;
-  phi_lj = eps4 * (s12 - s6);
-  f_lj = -eps4 * ((TC)12 * s12 - (TC)6 * s6) * rinv2;
-  const TC hl = eps4 * ((TC)168 * s12 - (TC)48 * s6) * rinv2 * rinv2;
-  const TC sginv = sg > (TC)0 ? (TC)1 / sg : (TC)0;
-  dps = (TC)0.5 * eps4 * ((TC)12 * s12 - (TC)6 * s6) * sginv;
-  dfs = (TC)-0.5 * eps4 * ((TC)144 * s12 - (TC)36 * s6) * rinv2 * sginv;
-  f = fc + f_lj;
-  h = hc + hl;
+// the value of the lane whose index within its eight differs in bit B (the partner for B = 2 is the
+// mirrored lane, whose lower bits differ too: only the opposite bit 2 matters below)
+template <int B>
+__device__ __forceinline__ float octet_xchg(float v) {
+  return dpp_f<B == 2 ? 0x141 : B == 1 ? 0x4E : 0xB1>(v);
+}
+template <int B>
+__device__ __forceinline__ double octet_xchg(double v) {
+  return __shfl_xor(v, 1 << B, TFF_WAVE);
+}
+template <bool STRIDE8, int B, typename T>
+__device__ __forceinline__ T xchg8(T v) {
+  if constexpr (STRIDE8)
+    return __shfl_xor(v, 8 << B, TFF_WAVE);
+  else
+    return octet_xchg<B>(v);
+}
+// Reduce-scatter of 24 values over eight lanes (contiguous lanes, or lanes at stride 8): lane m of
+// the eight ends with the sums of values m, m + 8 and m + 16.  Each of the three exchange steps
+// halves what a lane keeps: 21 exchanges where summing every value in every lane takes 72.
+template <bool STRIDE8, typename T>
+__device__ __forceinline__ void reduce_scatter24(const T (&a)[24], int m, T (&out)[3]) {
+  const bool b2 = (m >> 2) & 1, b1 = (m >> 1) & 1, b0 = m & 1;
+  T h[12];  // h[4 r + c]: value 8 r + 4 b2 + c
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const T lo = a[8 * r + c], hi = a[8 * r + 4 + c];
+      h[4 * r + c] = (b2 ? hi : lo) + xchg8<STRIDE8, 2>(b2 ? lo : hi);
+    }
+  T q[6];  // q[2 r + c]: value 8 r + 4 b2 + 2 b1 + c
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const T lo = h[4 * r + c], hi = h[4 * r + 2 + c];
+      q[2 * r + c] = (b1 ? hi : lo) + xchg8<STRIDE8, 1>(b1 ? lo : hi);
+    }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const T lo = q[2 * r], hi = q[2 * r + 1];
+    out[r] = (b0 ? hi : lo) + xchg8<STRIDE8, 0>(b0 ? lo : hi);
+  }
 }
 
 // One wave per workgroup over a contiguous run of the tile list (as the forward pair kernel);
 // one i-group at a time (lane = (ii, jj) holds i-slot gi * 8 + ii against j-slot gj * 8 + jj),
 // the tile's periodic shift (minimum image per pair on the tiles flagged for it).
+// Per pair, in factored form without divisions (Q = q_i q_j, s6 = (sigma_ij / r)^6):
+//   f = -k Q g / r^3 - 4 eps s6 (12 s6 - 6) / r^2
+//   h =  k Q (3 g / r^3 + (4/sqrt pi) a^3 e^{-a^2 r^2}) / r^2 + 4 eps s6 (168 s6 - 48) / r^4
+//   d phi / d sigma_ij = 4 eps (s6 / sigma_ij) (12 s6 - 6),  s6 / sigma_ij = s^4 sigma_ij / r^2,
+//   d (f s) / d sigma_ij = -4 eps (s6 / sigma_ij) (144 s6 - 36) s / r^2,
+// and 1 / (2 eps) of every slot precomputed (the w component of the per-slot upstream force).
+// The i-side sums stay in registers while ci repeats (the list is ci-ordered) and are flushed once
+// per run of tiles; the j-side sums once per tile.  Value q = 6 g + k of a cluster's 24 per lane
+// column is component k of slot g * 8 + column.
 template <typename TC>
 __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict__ cpairs,
                                                          const int* __restrict__ ncp_ptr, long long cap,
@@ -1048,12 +1075,38 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
   const long long t0 = gw * per;
   const long long t1 = t0 + per < ncp ? t0 + per : ncp;
   const TC gE = gE_ptr ? (TC)gE_ptr[0] : (TC)0;
+  const TC c2 = (TC)1.1283791670955126;                     // 2 / sqrt(pi)
+  const TC ch = (TC)2 * c2 * ta.alpha * ta.alpha * ta.alpha;  // (4 / sqrt pi) a^3
+  TC ai[4][GN];
+  int cur_i = -1;
+  // the 24 sums of a cluster's lane column over the eight lanes sharing it, added to the
+  // accumulators of slot (q / GN) * 8 + col (value q = col8 + 8 r of the lane at col8)
+  auto flush = [&](const TC (&a)[4][GN], int c, int col, int col8, auto stride8) {
+    TC v[3];
+    reduce_scatter24<decltype(stride8)::value>(reinterpret_cast<const TC(&)[24]>(a), col8, v);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int q = col8 + 8 * r;
+      if (v[r] != (TC)0) {
+        const long long sl = (long long)c * CL + (q / GN) * 8 + col;
+        atomicAdd(&gacc[GN * sl + q % GN], (unsigned long long)to_fx((double)v[r], F_SCALE));
+      }
+    }
+  };
   for (long long t = t0; t < t1; ++t) {
     const int4 cp = cpairs[t];
     const int ci = cp.x, cj = cp.y;
     const unsigned info = (unsigned)cp.z;
     const unsigned mask = info & 0xffffu;
     if (mask == 0) continue;
+    if (ci != cur_i) {
+      if (cur_i >= 0) flush(ai, cur_i, ii, jj, std::true_type{});
+      cur_i = ci;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int k = 0; k < GN; ++k) ai[g][k] = 0;
+    }
     const bool upper = ci != cj || jj > ii;
     // tiles flagged for per-pair minimum image keep it; the others take the tile's periodic shift
     const bool minimg = (info & INFO_MINIMG) != 0u;
@@ -1062,7 +1115,7 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
     const TC shz = minimg ? (TC)0 : ta.L[2] * (TC)((int)((info >> 20) & 3u) - 1);
     V4<TC> pj[4], vj[4];
     V2<TC> lJ[4];
-    TC ej_inv2[4];  // 1 / (2 eps_j)
+    TC kqj[4];
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int sl = cj * CL + g * 8 + jj;
@@ -1072,7 +1125,7 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
       pj[g].z += shz;
       lJ[g] = lj[sl];
       vj[g] = vsl[sl];
-      ej_inv2[g] = lJ[g].y > (TC)0 ? (TC)2 / (lJ[g].y * lJ[g].y) : (TC)0;
+      kqj[g] = ta.coul * pj[g].w;
     }
     TC aj[4][GN];
 #pragma unroll
@@ -1094,11 +1147,8 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
           if (pv >= 0 && (pv >> 5) == cj) exm |= 1u << (pv & 31);
         }
       }
-      TC ai[GN];
-#pragma unroll
-      for (int k = 0; k < GN; ++k) ai[k] = 0;
+      TC (&a)[GN] = ai[gi];
       const TC kqi = ta.coul * pi.w;
-      const TC ei_inv2 = li.y > (TC)0 ? (TC)2 / (li.y * li.y) : (TC)0;  // 1 / (2 eps_i)
 #pragma unroll
       for (int gj = 0; gj < 4; ++gj) {
         if (!((mask >> (gi * 4 + gj)) & 1u)) continue;
@@ -1114,62 +1164,52 @@ __global__ __launch_bounds__(TFF_WAVE) void k_tiles_grad(const int4* __restrict_
         if (__builtin_amdgcn_ballot_w64(ok) == 0) continue;  // no pair of the sub-tile in range
         const bool excl = (exm >> (gj * 8 + jj)) & 1u;
         r2 = ok ? r2 : (TC)1e30;
-        TC f, h, phi_c, dpqi, dpqj, dfqi, dfqj, phi_lj, f_lj, dps, dfs;
-        pair_grad_terms<TC>(r2, excl, !excl, kqi, pi.w, pj[gj].w, li.x, lJ[gj].x, li.y, lJ[gj].y, ta, f, h, phi_c,
-                            dpqi, dpqj, dfqi, dfqj, phi_lj, f_lj, dps, dfs);
+        const TC rinv = rsqrt_t(r2);
+        const TC rinv2 = rinv * rinv;
+        const TC u = ta.alpha * (r2 * rinv);
+        TC ec, em;
+        erfc_exp(u, ec, em);
+        ec = excl ? ec - (TC)1 : ec;                       // c = erfc - [excluded]
+        const TC grinv3 = fma(c2 * u, em, ec) * rinv * rinv2;  // g / r^3
+        const TC kQ = kqi * pj[gj].w;
+        const TC eps4 = excl ? (TC)0 : li.y * lJ[gj].y;
+        const TC sg = li.x + lJ[gj].x;
+        const TC tq = sg * rinv2;  // s^2 / sigma_ij
+        const TC s2 = sg * tq;
+        const TC s4 = s2 * s2;
+        const TC s6 = s4 * s2;
+        const TC ps = s4 * tq;     // s6 / sigma_ij
+        const TC a12 = fma((TC)12, s6, (TC)-6);
+        const TC e6 = eps4 * s6;
+        const TC f = fma(-kQ, grinv3, -e6 * a12 * rinv2);
+        const TC h = fma(kQ * rinv2, fma((TC)3, grinv3, ch * em), e6 * fma((TC)168, s6, (TC)-48) * (rinv2 * rinv2));
         const TC okf = ok ? (TC)1 : (TC)0;
         const TC dvx = vi.x - vj[gj].x, dvy = vi.y - vj[gj].y, dvz = vi.z - vj[gj].z;
         const TC sdv = dx * dvx + dy * dvy + dz * dvz;
-        const TC cd = okf * (gE * f - sdv * h), cv = okf * f;
+        const TC cd = okf * fma(gE, f, -sdv * h), cv = okf * f;
         const TC gx = cd * dx - cv * dvx, gy = cd * dy - cv * dvy, gz = cd * dz - cv * dvz;
-        ai[0] += gx;
-        ai[1] += gy;
-        ai[2] += gz;
+        a[0] += gx;
+        a[1] += gy;
+        a[2] += gz;
         aj[gj][0] -= gx;
         aj[gj][1] -= gy;
         aj[gj][2] -= gz;
-        ai[3] += okf * (gE * dpqi - sdv * dfqi);
-        aj[gj][3] += okf * (gE * dpqj - sdv * dfqj);
-        const TC ts = okf * (gE * dps - sdv * dfs);
-        ai[4] += ts;
+        // d/dq_i = k q_j (gE c / r + (s.dv) g / r^3), d/dq_j alike with k q_i
+        const TC gq = okf * fma(gE, ec * rinv, sdv * grinv3);
+        a[3] = fma(kqj[gj], gq, a[3]);
+        aj[gj][3] = fma(kqi, gq, aj[gj][3]);
+        const TC rs = sdv * rinv2;
+        const TC ts = okf * (TC)0.5 * eps4 * ps * fma(gE, a12, rs * fma((TC)144, s6, (TC)-36));
+        a[4] += ts;
         aj[gj][4] += ts;
-        const TC te = okf * (gE * phi_lj - sdv * f_lj);
-        ai[5] += te * ei_inv2;
-        aj[gj][5] += te * ej_inv2[gj];
-        (void)phi_c;
-      }
-      // i-slot sums over the 8 lanes sharing ii; lane jj < GN writes value jj
-      TC mine = 0;
-#pragma unroll
-      for (int k = 0; k < GN; ++k) {
-        const TC sk = stride8_sum(ai[k]);
-        mine = jj == k ? sk : mine;
-      }
-      if (jj < GN && mine != (TC)0) atomicAdd(&gacc[(long long)GN * sli + jj], (unsigned long long)to_fx((double)mine, F_SCALE));
-    }
-    // j-slot sums over the 8 lanes sharing jj; lane ii writes values ii, ii + 8, ii + 16 (< 24)
-#pragma unroll
-    for (int g = 0; g < 4; ++g)
-#pragma unroll
-      for (int k = 0; k < GN; ++k) aj[g][k] = octet_sum(aj[g][k]);
-    TC w0 = 0, w1 = 0, w2 = 0;
-#pragma unroll
-    for (int q = 0; q < 4 * GN; ++q) {
-      const TC vq = aj[q / GN][q % GN];
-      w0 = ii == q ? vq : w0;
-      w1 = ii + 8 == q ? vq : w1;
-      w2 = ii + 16 == q ? vq : w2;
-    }
-    const TC wv[3] = {w0, w1, w2};
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const int q = ii + 8 * r;
-      if (q < 4 * GN && wv[r] != (TC)0) {
-        const int slj = cj * CL + (q / GN) * 8 + jj;
-        atomicAdd(&gacc[(long long)GN * slj + q % GN], (unsigned long long)to_fx((double)wv[r], F_SCALE));
+        const TC te = okf * e6 * fma(gE, s6 - (TC)1, rs * a12);
+        a[5] = fma(te, vi.w, a[5]);
+        aj[gj][5] = fma(te, vj[gj].w, aj[gj][5]);
       }
     }
+    flush(aj, cj, jj, ii, std::false_type{});
   }
+  if (cur_i >= 0) flush(ai, cur_i, ii, jj, std::true_type{});
 }
 
 // ---------------------------------------------------------------- 5. PME on cluster slots
@@ -2412,7 +2452,7 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
                    const Geo& g, const int64_t* row_ptr, const int64_t* col, double cutoff, double list_cutoff,
                    double alpha, double coul, const int K[3], int order, const TC* mx, const TC* my, const TC* mz,
                    const double* gE, const TI* gF, TI* dr, TI* dq, TI* dsig, TI* deps, int* status, const TI* box_dev,
-                   int* status_host, const FusedWs& w, hipStream_t st) {
+                   int* status_host, const FusedWs& w, bool reuse, hipStream_t st) {
   if (order < 3 || order > 8) {
     tff_set_error("fused: unsupported spline order %d (3..8)", order);
     return TFF_ERR_UNSUPPORTED;
@@ -2428,26 +2468,36 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
       cl.n[cl.count] = words;
       ++cl.count;
     };
-    add(w.col_count, ncols + 1);
-    add(w.ncp, 4);
-    add(status, 2);
+    if (!reuse) {
+      add(w.col_count, ncols + 1);
+      add(w.ncp, 4);
+      add(status, 2);
+      add(w.slot_orig, nslots);
+      // the brick counters of the charge spread (float: the forward's list spread runs again)
+      if (w.bcount) add(w.bcount, (long long)w.bg.nb[0] * w.bg.nb[1] * w.bg.nb[2]);
+    }
     add(w.gacc, 2LL * GN * nslots);
-    add(w.slot_orig, nslots);
     hipLaunchKernelGGL(k_clear, dim3(1024), dim3(256), 0, st, cl);
   }
-  hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, q, n, g, w.col_of, w.off_in_col,
-                     w.col_count, w.qparts, box_dev, status);
-  hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.col_count, ncols, w.col_start, w.col_cl_start,
-                     w.ncl, status);
-  hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
-                     w.sorted_orig);
-  hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(SORT_THREADS), 0, st, coords, g, w.col_start, w.sorted_orig);
   const unsigned nbW = (unsigned)(((long long)g.ncl_max * TFF_WAVE + FB - 1) / FB);
-  const unsigned nbB = (unsigned)(((long long)g.ncl_max * CL + FB - 1) / FB);  // two clusters per wave
-  hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbB), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
-                     w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
-                     w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
-  hipLaunchKernelGGL((k_fill_vslots<TI, TC>), dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, gF, (V4<TC>*)w.vsl);
+  // reuse: the cluster arrays, the exclusion table and the pair list are those the forward call
+  // left in the workspace (for the same inputs); otherwise they are rebuilt here
+  if (!reuse) {
+    hipLaunchKernelGGL(k_bin_atoms<TI>, dim3(nbA), dim3(FB), 0, st, coords, q, n, g, w.col_of, w.off_in_col,
+                       w.col_count, w.qparts, box_dev, status);
+    hipLaunchKernelGGL(k_scan_columns, dim3(1), dim3(1024), 0, st, w.col_count, ncols, w.col_start,
+                       w.col_cl_start, w.ncl, status);
+    hipLaunchKernelGGL(k_scatter_atoms, dim3(nbA), dim3(FB), 0, st, n, w.col_of, w.off_in_col, w.col_start,
+                       w.sorted_orig);
+    hipLaunchKernelGGL(k_sort_columns<TI>, dim3(ncols), dim3(SORT_THREADS), 0, st, coords, g, w.col_start,
+                       w.sorted_orig);
+    const unsigned nbB = (unsigned)(((long long)g.ncl_max * CL + FB - 1) / FB);  // two clusters per wave
+    hipLaunchKernelGGL((k_build_clusters<TI, TC>), dim3(nbB), dim3(FB), 0, st, coords, q, sig, eps, g, w.ncl,
+                       w.col_cl_start, w.col_start, w.sorted_orig, (V4<TC>*)w.xyzq, (V2<TC>*)w.lj, w.slot_orig,
+                       w.slot_of_orig, w.cl_col, w.bb_c, w.bb_h, w.sb_c, w.sb_h);
+  }
+  hipLaunchKernelGGL((k_fill_vslots<TI, TC>), dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, gF, (const V2<TC>*)w.lj,
+                     (V4<TC>*)w.vsl, reuse ? (const int*)w.st : nullptr, status);
   // the reciprocal-space part (charge and dipole grids, both transforms, the gather) needs only
   // the cluster arrays and the per-slot upstream forces: it runs on the auxiliary stream beside the
   // pair search and the pair kernel, joined before the final kernel
@@ -2456,45 +2506,17 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
     tff_set_error("fused backward: could not create the auxiliary stream");
     return TFF_ERR_LAUNCH;
   }
-  PmeGeom pg = make_geom(box, K, order, alpha);
-  FftPlan pl[3];
-  for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
-  const unsigned nbricks = (unsigned)(((K[0] + BR - 1) / BR) * ((K[1] + BR - 1) / BR) * ((K[2] + BR - 1) / BR));
-  V4<TC> jsc{(TC)(K[0] * box.inv[0]), (TC)(K[1] * box.inv[4]), (TC)(K[2] * box.inv[8]), (TC)0};
-  TC* phi = (TC*)w.grid;
-  TC* psi = (TC*)w.grid_d;
-  const unsigned nb = (unsigned)((nslots + FB - 1) / FB);
-#define TFF_GRAD_PME(P)                                                                                          \
-  {                                                                                                              \
-    hipLaunchKernelGGL((k_spread_bricks<TC, P, false>), dim3(nbricks), dim3(256), 0, sp, w.ncl,                  \
-                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid, status, \
-                       (const V4<TC>*)nullptr, V4<TC>{});                                                        \
-    hipLaunchKernelGGL((k_spread_bricks<TC, P, true>), dim3(nbricks), dim3(256), 0, sp, w.ncl,                   \
-                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid_d,       \
-                       status, (const V4<TC>*)w.vsl, jsc);                                                       \
-    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts,          \
-                      "fused_grad_fft", sp);                                                                     \
-    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid_d, 1.0 / Q_SCALE, psi, w.kparts,                 \
-                      "fused_grad_fft", sp);                                                                     \
-    hipLaunchKernelGGL((k_gather_grad<TC, P>), dim3(nb), dim3(FB), 0, sp, w.ncl, phi, psi, (const V4<TC>*)w.xyzq, \
-                       (const V4<TC>*)w.vsl, w.slot_orig, pg, coul, gE, w.gpme);                                 \
+  // the pair search and the pair kernel are enqueued first, so that the pair kernel starts as soon
+  // as the caller's stream reaches it rather than after the host has enqueued the reciprocal chain
+  if (!reuse) {
+    hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n,
+                       row_ptr, col, w.excl_part, status);
+    hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c,
+                       w.bb_h, g, (float)(list_cutoff * list_cutoff), w.excl_part, w.seg, w.seg_count, status);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
+    hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
+                       w.sb_c, w.sb_h, g, (float)(cutoff * cutoff), w.cpairs);
   }
-  switch (order) {
-    case 3: TFF_GRAD_PME(3); break;
-    case 4: TFF_GRAD_PME(4); break;
-    case 5: TFF_GRAD_PME(5); break;
-    case 6: TFF_GRAD_PME(6); break;
-    case 7: TFF_GRAD_PME(7); break;
-    default: TFF_GRAD_PME(8); break;
-  }
-#undef TFF_GRAD_PME
-  hipLaunchKernelGGL(k_fill_exclusions, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.slot_of_orig, n, row_ptr,
-                     col, w.excl_part, status);
-  hipLaunchKernelGGL(k_cluster_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.cl_col, w.col_cl_start, w.bb_c, w.bb_h,
-                     g, (float)(list_cutoff * list_cutoff), w.excl_part, w.seg, w.seg_count, status);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, w.ncl, w.seg_count, w.seg_start, w.ncp, status);
-  hipLaunchKernelGGL(k_compact_pairs, dim3(nbW), dim3(FB), 0, st, w.ncl, w.seg, w.seg_count, w.seg_start, g.maxn,
-                     w.sb_c, w.sb_h, g, (float)(cutoff * cutoff), w.cpairs);
   {
     TileArgs<TC> ta;
     for (int a = 0; a < 3; ++a) {
@@ -2511,6 +2533,44 @@ int run_fused_grad(const TI* coords, const TI* q, const TI* sig, const TI* eps, 
                        gE, w.gacc);
     tff_prof_end(tok, st);
   }
+  PmeGeom pg = make_geom(box, K, order, alpha);
+  FftPlan pl[3];
+  for (int a = 0; a < 3; ++a) make_plan(K[a], pl[a]);
+  const unsigned nbricks = (unsigned)(((K[0] + BR - 1) / BR) * ((K[1] + BR - 1) / BR) * ((K[2] + BR - 1) / BR));
+  V4<TC> jsc{(TC)(K[0] * box.inv[0]), (TC)(K[1] * box.inv[4]), (TC)(K[2] * box.inv[8]), (TC)0};
+  TC* phi = (TC*)w.grid;
+  TC* psi = (TC*)w.grid_d;
+  const unsigned nb = (unsigned)((nslots + FB - 1) / FB);
+#define TFF_GRAD_PME(P)                                                                                          \
+  {                                                                                                              \
+    hipLaunchKernelGGL((k_spread_bricks<TC, P, true>), dim3(nbricks), dim3(256), 0, sp, w.ncl,                   \
+                       (const V4<TC>*)w.xyzq, w.slot_orig, w.bb_c, w.bb_h, w.col_cl_start, g, pg, w.grid_d,       \
+                       status, (const V4<TC>*)w.vsl, jsc);                                                       \
+    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid_d, 1.0 / Q_SCALE, psi, w.kparts,                 \
+                      "fused_grad_fft", sp);                                                                     \
+    hipLaunchKernelGGL((k_gather_grad<TC, P>), dim3(nb), dim3(FB), 0, sp, w.ncl, phi, psi, (const V4<TC>*)w.xyzq, \
+                       (const V4<TC>*)w.vsl, w.slot_orig, pg, coul, gE, w.gpme);                                 \
+  }
+  // the potential grid phi of the charges: the forward call's, left in place (reuse), or made
+  // again by the forward's own spread and transforms, so that both give the same bits
+  if (!reuse) {
+    const int rc = launch_pme_slots<TC>(order, w, g, pg, nslots, (const TC*)nullptr, coul, true, status, sp);
+    if (rc) {
+      aux_stream_join(st, sp);
+      return rc;
+    }
+    run_recip_fft<TC>(w, K, box, alpha, pl, mx, my, mz, w.grid, 1.0 / grid_scale<TC>(), phi, w.kparts,
+                      "fused_grad_fft", sp);
+  }
+  switch (order) {
+    case 3: TFF_GRAD_PME(3); break;
+    case 4: TFF_GRAD_PME(4); break;
+    case 5: TFF_GRAD_PME(5); break;
+    case 6: TFF_GRAD_PME(6); break;
+    case 7: TFF_GRAD_PME(7); break;
+    default: TFF_GRAD_PME(8); break;
+  }
+#undef TFF_GRAD_PME
   aux_stream_join(st, sp);
   hipLaunchKernelGGL(k_finalize_grad<TI>, dim3(nbS), dim3(FB), 0, st, w.ncl, w.slot_orig, w.gacc, w.gpme, q, coul,
                      alpha, gE, dr, dq, dsig, deps, status, status_host);
@@ -2573,14 +2633,15 @@ extern "C" size_t tff_fused_backward_workspace_bytes(int compute_dtype, long lon
   return carve(nullptr, natoms, g, K, compute_dtype == TFF_F64 ? 8 : 4, true).bytes;
 }
 
-extern "C" int tff_fused_energy_forces_backward(
-    int io_dtype, int compute_dtype, const void* coords, const void* charges, const void* sigma, const void* epsilon,
-    long long natoms, const double* box_host, const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff,
-    double list_cutoff, double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
-    const void* xmoduli, const void* ymoduli, const void* zmoduli, const double* grad_energy,
-    const void* grad_forces, void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon,
-    int32_t* status_out, const void* box_device, int32_t* status_host, void* workspace, size_t workspace_bytes,
-    hipStream_t stream) {
+namespace {
+int fused_backward(int io_dtype, int compute_dtype, const void* coords, const void* charges, const void* sigma,
+                   const void* epsilon, long long natoms, const double* box_host, const int64_t* excl_row_ptr,
+                   const int64_t* excl_col, double cutoff, double list_cutoff, double ewald_alpha,
+                   double coul_constant, int K0, int K1, int K2, int order, const void* xmoduli,
+                   const void* ymoduli, const void* zmoduli, const double* grad_energy, const void* grad_forces,
+                   void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon, int32_t* status_out,
+                   const void* box_device, int32_t* status_host, void* workspace, size_t workspace_bytes,
+                   bool reuse, hipStream_t stream) {
   TffBox box = tff_make_box(box_host);
   Geo g;
   int K[3] = {K0, K1, K2};
@@ -2603,7 +2664,7 @@ extern "C" int tff_fused_energy_forces_backward(
                                 coul_constant, K, order, (const TC*)xmoduli, (const TC*)ymoduli,                  \
                                 (const TC*)zmoduli, grad_energy, (const TI*)grad_forces, (TI*)grad_coords,         \
                                 (TI*)grad_charges, (TI*)grad_sigma, (TI*)grad_epsilon, (int*)status_out,        \
-                                (const TI*)box_device, (int*)status_host, w, stream)
+                                (const TI*)box_device, (int*)status_host, w, reuse, stream)
   if (io_dtype == TFF_F64 && compute_dtype == TFF_F32) TFF_RUN(double, float);
   if (io_dtype == TFF_F32 && compute_dtype == TFF_F32) TFF_RUN(float, float);
   if (io_dtype == TFF_F64 && compute_dtype == TFF_F64) TFF_RUN(double, double);
@@ -2611,6 +2672,35 @@ extern "C" int tff_fused_energy_forces_backward(
 #undef TFF_RUN
   tff_set_error("fused backward: unsupported dtype combination");
   return TFF_ERR_ARG;
+}
+}  // namespace
+
+extern "C" int tff_fused_energy_forces_backward(
+    int io_dtype, int compute_dtype, const void* coords, const void* charges, const void* sigma, const void* epsilon,
+    long long natoms, const double* box_host, const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff,
+    double list_cutoff, double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
+    const void* xmoduli, const void* ymoduli, const void* zmoduli, const double* grad_energy,
+    const void* grad_forces, void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon,
+    int32_t* status_out, const void* box_device, int32_t* status_host, void* workspace, size_t workspace_bytes,
+    hipStream_t stream) {
+  return fused_backward(io_dtype, compute_dtype, coords, charges, sigma, epsilon, natoms, box_host, excl_row_ptr,
+                        excl_col, cutoff, list_cutoff, ewald_alpha, coul_constant, K0, K1, K2, order, xmoduli,
+                        ymoduli, zmoduli, grad_energy, grad_forces, grad_coords, grad_charges, grad_sigma,
+                        grad_epsilon, status_out, box_device, status_host, workspace, workspace_bytes, false, stream);
+}
+
+extern "C" int tff_fused_energy_forces_backward_reuse(
+    int io_dtype, int compute_dtype, const void* coords, const void* charges, const void* sigma, const void* epsilon,
+    long long natoms, const double* box_host, const int64_t* excl_row_ptr, const int64_t* excl_col, double cutoff,
+    double list_cutoff, double ewald_alpha, double coul_constant, int K0, int K1, int K2, int order,
+    const void* xmoduli, const void* ymoduli, const void* zmoduli, const double* grad_energy,
+    const void* grad_forces, void* grad_coords, void* grad_charges, void* grad_sigma, void* grad_epsilon,
+    int32_t* status_out, const void* box_device, int32_t* status_host, void* workspace, size_t workspace_bytes,
+    hipStream_t stream) {
+  return fused_backward(io_dtype, compute_dtype, coords, charges, sigma, epsilon, natoms, box_host, excl_row_ptr,
+                        excl_col, cutoff, list_cutoff, ewald_alpha, coul_constant, K0, K1, K2, order, xmoduli,
+                        ymoduli, zmoduli, grad_energy, grad_forces, grad_coords, grad_charges, grad_sigma,
+                        grad_epsilon, status_out, box_device, status_host, workspace, workspace_bytes, true, stream);
 }
 
 namespace {

@@ -67,6 +67,9 @@ _SIGNATURES = {
     "tff_fused_energy_forces_backward": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d, _i, _i,
                                               _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                               _vp, _sz, _vp]),
+    "tff_fused_energy_forces_backward_reuse": (_i, [_i, _i, _vp, _vp, _vp, _vp, _ll, _vp, _vp, _vp, _d, _d, _d, _d,
+                                                    _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                    _vp, _vp, _vp, _vp, _sz, _vp]),
     "tff_serve_table_bytes": (_sz, [_i, _i]),
     "tff_serve_atoms_offset": (_sz, [_i]),
     "tff_serve_table_reset": (None, [_vp]),

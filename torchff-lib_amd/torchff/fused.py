@@ -16,8 +16,11 @@ Differentiation: when an input requires grad, (E, F) are differentiable with res
 coords, charges, sigma and epsilon through a hand-written vector-Jacobian product
 (``tff_fused_energy_forces_backward``): a loss of the forces (force matching) gets
 gE dE/dtheta - d(sum_i v_i . dE/dr_i)/dtheta with v = dL/dF, i.e. the double backward the
-reference computes with create_graph on its pure-PyTorch path.  The backward rebuilds the
-cluster pair list; it is not itself differentiable.
+reference computes with create_graph on its pure-PyTorch path.  Such a forward call runs in
+the backward's (larger) workspace, and its backward takes the cluster arrays and pair list the
+forward left there (``tff_fused_energy_forces_backward_reuse``) when nothing has written that
+workspace since, on the same stream; otherwise it rebuilds them.  The backward is not itself
+differentiable.
 """
 
 import math
@@ -99,6 +102,12 @@ class FusedNonbondedPME(nn.Module):
         self._ring_dev = None
         self._pending = deque()
         self._captured = False  # a call was captured into a CUDA graph (its replays write ring row _RING)
+        # the backward-sized workspace: writes to it are counted, so that a backward can tell whether
+        # the cluster arrays and pair list of its forward call are still there
+        self._bws = None
+        self._bws_key = None
+        self._bws_gen = 0
+        self._last_token = None
 
     def _workspace(self, box_host, device):
         lib = L.load()
@@ -111,6 +120,19 @@ class FusedNonbondedPME(nn.Module):
             self._ws = L.workspace(nbytes, device)
             self._ws_key = key
         return self._ws
+
+    def _backward_workspace(self, box_host, device):
+        lib = L.load()
+        cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
+        key = (tuple(box_host), device)
+        if self._bws is None or self._bws_key != key:
+            nbytes = lib.tff_fused_backward_workspace_bytes(cd, self.natoms, box_host, self.list_cutoff, *self.K)
+            if nbytes == 0:
+                raise RuntimeError("FusedNonbondedPME needs an orthorhombic periodic box")
+            self._bws = L.workspace(nbytes, device)
+            self._bws_key = key
+            self._bws_gen += 1
+        return self._bws
 
     def _status_slot(self, device):
         """(device status tensor, pinned host mirror) for a new call, after reading the status of
@@ -194,7 +216,9 @@ class FusedNonbondedPME(nn.Module):
             return E, F, terms
         return self._forward_raw(coords, box, charges, sigma, epsilon, check)
 
-    def _forward_raw(self, coords, box, charges, sigma, epsilon, check=False):
+    def _forward_raw(self, coords, box, charges, sigma, epsilon, check=False, for_grad=False):
+        """for_grad: run in the backward's workspace and leave in _last_token what its backward
+        needs to take the cluster arrays and pair list from there (None under graph capture)."""
         L.require_cuda(coords, box, charges, sigma, epsilon)
         lib = L.load()
         dev = coords.device
@@ -204,7 +228,7 @@ class FusedNonbondedPME(nn.Module):
             raise RuntimeError(f"coords must be ({self.natoms}, 3)")
         q, s, e = (t.detach().to(io).contiguous() for t in (charges, sigma, epsilon))
         bh = L.box_host(box)
-        ws = self._workspace(bh, dev)
+        ws = self._backward_workspace(bh, dev) if for_grad else self._workspace(bh, dev)
         if self._mod is None or self._mod[0].device != dev or self._mod[0].dtype != self.compute_dtype:
             self._mod = tuple(m.to(device=dev, dtype=self.compute_dtype).contiguous()
                               for m in (self.xmoduli, self.ymoduli, self.zmoduli))
@@ -224,15 +248,22 @@ class FusedNonbondedPME(nn.Module):
                                          L.ptr(out), L.ptr(forces), L.ptr(status), L.ptr(bd), L.ptr(host),
                                          L.ptr(ws), ws.numel(), L.stream_of(c))
         L.check(rc, "FusedNonbondedPME")
+        if for_grad:
+            self._bws_gen += 1
+            capturing = torch.cuda.is_current_stream_capturing()
+            self._last_token = None if capturing else (self._bws_gen, L.stream_of(c).value)
         self._record(row, "forward call")
         self.status = status
         if check:
             self.check()
         return out[4], forces, out[:4]
 
-    def _backward_raw(self, coords, box, charges, sigma, epsilon, grad_energy, grad_forces, needs):
+    def _backward_raw(self, coords, box, charges, sigma, epsilon, grad_energy, grad_forces, needs, token=None):
         """dL/d(coords, charges, sigma, epsilon) for upstream grad_energy (0-d, or None) and
-        grad_forces ((N, 3), or None); entries of ``needs`` False give None."""
+        grad_forces ((N, 3), or None); entries of ``needs`` False give None.  token: the forward
+        call's (from _last_token): while no call has written the backward workspace since, on the
+        same stream and with no call of this module captured into a graph (whose replays write it
+        unseen), the forward's cluster arrays and pair list are reused."""
         lib = L.load()
         dev = coords.device
         io = coords.dtype
@@ -240,11 +271,10 @@ class FusedNonbondedPME(nn.Module):
         q, s, e = (t.detach().to(io).contiguous() for t in (charges, sigma, epsilon))
         bh = L.box_host(box)
         cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
-        key = (tuple(bh), dev)
-        if getattr(self, "_bws", None) is None or self._bws_key != key:
-            nbytes = lib.tff_fused_backward_workspace_bytes(cd, self.natoms, bh, self.list_cutoff, *self.K)
-            self._bws = L.workspace(nbytes, dev)
-            self._bws_key = key
+        ws = self._backward_workspace(bh, dev)
+        stream = L.stream_of(c)
+        reuse = (token is not None and not self._captured and token == (self._bws_gen, stream.value)
+                 and not torch.cuda.is_current_stream_capturing())
         ge = None if grad_energy is None else grad_energy.detach().to(device=dev, dtype=torch.float64).reshape(1)
         gf = None if grad_forces is None else grad_forces.detach().to(device=dev, dtype=io).contiguous()
         outs = [torch.empty_like(c) if needs[0] else None] + \
@@ -252,12 +282,15 @@ class FusedNonbondedPME(nn.Module):
         status, host, row = self._status_slot(dev)
         bd = self._box_device(box, io)
         rp, col = self.excl_row_ptr, self.excl_col_indices
-        rc = lib.tff_fused_energy_forces_backward(
+        entry = lib.tff_fused_energy_forces_backward_reuse if reuse else lib.tff_fused_energy_forces_backward
+        rc = entry(
             L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms, bh, L.ptr(rp), L.ptr(col),
             self.cutoff, self.list_cutoff, self.alpha, self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
             L.ptr(ge), L.ptr(gf), *(L.ptr(o) for o in outs), L.ptr(status), L.ptr(bd), L.ptr(host),
-            L.ptr(self._bws), self._bws.numel(), L.stream_of(c))
+            L.ptr(ws), ws.numel(), stream)
         L.check(rc, "FusedNonbondedPME backward")
+        if not reuse:
+            self._bws_gen += 1  # rebuilt in place
         self._record(row, "backward call")
         self.backward_status = status
         return outs
@@ -288,7 +321,8 @@ class _FusedEnergyForces(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, module, coords, charges, sigma, epsilon, box):
-        E, F, terms = module._forward_raw(coords, box, charges, sigma, epsilon)
+        E, F, terms = module._forward_raw(coords, box, charges, sigma, epsilon, for_grad=True)
+        ctx.token = module._last_token
         ctx.module = module
         ctx.save_for_backward(coords, charges, sigma, epsilon, box)
         terms = terms.clone()
@@ -302,7 +336,8 @@ class _FusedEnergyForces(torch.autograd.Function):
         needs = ctx.needs_input_grad[1:5]
         if not any(needs):
             return None, None, None, None, None, None
-        d = ctx.module._backward_raw(coords, box, charges, sigma, epsilon, grad_energy, grad_forces, needs)
+        d = ctx.module._backward_raw(coords, box, charges, sigma, epsilon, grad_energy, grad_forces, needs,
+                                     ctx.token)
         d = [None if x is None else x.to(t.dtype) for x, t in zip(d, (coords, charges, sigma, epsilon))]
         return (None, *d, None)
 
