@@ -100,6 +100,7 @@ class FusedNonbondedPME(nn.Module):
         self._ring_np = None
         self._ring_next = 0
         self._ring_dev = None
+        self._ring_dstat = {}  # device -> (_RING + 1, 2) int32: each row the status words of its call
         self._pending = deque()
         self._captured = False  # a call was captured into a CUDA graph (its replays write ring row _RING)
         # the backward-sized workspace: writes to it are counted, so that a backward can tell whether
@@ -154,7 +155,14 @@ class FusedNonbondedPME(nn.Module):
             row = self._ring_next
             self._ring_next = (row + 1) % _RING
         self._ring_np[row] = -1
-        return torch.empty(2, dtype=torch.int32, device=device), self._ring[row], row
+        dstat = self._ring_dstat.get(device)
+        if dstat is None:
+            if capturing:  # allocated outside any capture only
+                return torch.empty(2, dtype=torch.int32, device=device), self._ring[row], row
+            dstat = self._ring_dstat[device] = torch.empty((_RING + 1, 2), dtype=torch.int32, device=device)
+        # a row is reused only after its earlier call has completed (at most _RING calls are
+        # outstanding, and the oldest is waited for first)
+        return dstat[row], self._ring[row], row
 
     def _record(self, row, what):
         if not torch.cuda.is_current_stream_capturing():
@@ -277,8 +285,13 @@ class FusedNonbondedPME(nn.Module):
                  and not torch.cuda.is_current_stream_capturing())
         ge = None if grad_energy is None else grad_energy.detach().to(device=dev, dtype=torch.float64).reshape(1)
         gf = None if grad_forces is None else grad_forces.detach().to(device=dev, dtype=io).contiguous()
-        outs = [torch.empty_like(c) if needs[0] else None] + \
-               [torch.empty(self.natoms, dtype=io, device=dev) if n else None for n in needs[1:]]
+        # the requested gradients as views of one allocation
+        sizes = [3 * self.natoms if needs[0] else 0] + [self.natoms if n else 0 for n in needs[1:]]
+        buf = torch.empty(sum(sizes), dtype=io, device=dev)
+        outs, o = [], 0
+        for k, n in enumerate(sizes):
+            outs.append(None if n == 0 else (buf[o:o + n].view(self.natoms, 3) if k == 0 else buf[o:o + n]))
+            o += n
         status, host, row = self._status_slot(dev)
         bd = self._box_device(box, io)
         rp, col = self.excl_row_ptr, self.excl_col_indices
@@ -325,9 +338,9 @@ class _FusedEnergyForces(torch.autograd.Function):
         ctx.token = module._last_token
         ctx.module = module
         ctx.save_for_backward(coords, charges, sigma, epsilon, box)
-        terms = terms.clone()
+        # E and terms are views of the call's 5-value output (no copies)
         ctx.mark_non_differentiable(terms)
-        return E.clone(), F, terms
+        return E, F, terms
 
     @staticmethod
     @torch.autograd.function.once_differentiable
