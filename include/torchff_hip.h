@@ -331,9 +331,10 @@ int tff_fused_energy_forces_backward(int io_dtype, int compute_dtype, const void
                                      void* grad_charges, void* grad_sigma, void* grad_epsilon,
                                      int32_t* status_out, const void* box_device, int32_t* status_host,
                                      void* workspace, size_t workspace_bytes, tff_stream_t stream);
-/* As tff_fused_energy_forces_backward, but the cluster arrays, the exclusion table and the pair
- * list are taken from the workspace as the last tff_fused_energy_forces call on it left them,
- * instead of being rebuilt (about 80 us of a 96 000-atom water box's backward).  The caller
+/* As tff_fused_energy_forces_backward, but the cluster arrays, the exclusion table, the pair list
+ * and the charges' potential grid are taken from the workspace as the last
+ * tff_fused_energy_forces call on it left them, instead of being made again (about 130 us of a
+ * force-matching step at 96 000 atoms); the results are bitwise the same.  The caller
  * guarantees that this call was made with the same (backward-sized) workspace, the same inputs
  * (coordinates, charges, sigma, epsilon, box, exclusions, cutoffs, grid) and on the same stream, and
  * that no other call has written the workspace since; the forward call's status words (capacity
