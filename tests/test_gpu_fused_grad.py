@@ -198,3 +198,34 @@ def test_backward_reuse_of_the_forward_list(case, precision):
         assert np.array_equal(a, b), k
     # the reused and the rebuilt results differ between the two coordinate sets (the test can fail)
     assert not np.array_equal(gA[0], gB[0])
+
+
+@pytest.mark.parametrize("order", [4, 8])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-4)])
+def test_force_matching_gradients_other_orders(case, order, precision, tol):
+    # spline orders 4 (the float list spread) and 8 (the brick spread) against the oracle, through
+    # the backward that reuses the forward's grid and list and through the one that rebuilds them
+    # (a second forward of the same inputs between a forward and its backward)
+    w, q, s, e, T, pairs = case
+    w2 = dict(w, order=order)
+    L_ref, _, g_ref = composite.force_matching(w2, q, s, e, T, pairs=pairs)
+    m = _module(w2, precision)
+    Tt = torch.tensor(T, device="cuda")
+    with _EntryCounter() as cnt:
+        inp = _inputs(w2, q, s, e)
+        _, F, _ = m(*inp)
+        L = torch.mean((F - Tt) ** 2)
+        L.backward()
+        g_reuse = _grads(inp)
+        inp1, inp2 = _inputs(w2, q, s, e), _inputs(w2, q, s, e)
+        _, F1, _ = m(*inp1)
+        _, F2, _ = m(*inp2)
+        torch.mean((F1 - Tt) ** 2).backward()
+        g_rebuild = _grads(inp1)
+        assert cnt.counts() == (1, 1)
+    m.check()
+    assert abs(float(L) - L_ref) <= max(tol, 1e-10) * abs(L_ref)
+    for k, a, b in zip(NAMES, g_reuse, g_rebuild):
+        assert np.array_equal(a, b), k
+        err = _rms_rel(a, g_ref[k])
+        assert err < tol, (order, precision, k, err)
