@@ -229,3 +229,35 @@ def test_force_matching_gradients_other_orders(case, order, precision, tol):
         assert np.array_equal(a, b), k
         err = _rms_rel(a, g_ref[k])
         assert err < tol, (order, precision, k, err)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_graph_replayed_forward_then_reusing_backward(case, precision):
+    # forward calls replayed as a captured graph (mode 2: always, float64 too), each followed by the
+    # backward that reuses its list and grid, give the bits of launches made one by one (mode 0);
+    # three calls: the key first seen, captured and replayed, replayed
+    from torchff import _lib as L
+    lib = L.load()
+    w, q, s, e, T, pairs = case
+    m = _module(w, precision)
+    Tt = torch.tensor(T, device="cuda")
+
+    def grads():
+        inp = _inputs(w, q, s, e)
+        _, F, _ = m(*inp)
+        torch.mean((F - Tt) ** 2).backward()
+        return _grads(inp)
+
+    try:
+        lib.tff_fused_set_graphs(0)
+        ref = grads()
+        lib.tff_fused_set_graphs(2)
+        with _EntryCounter() as cnt:
+            got = [grads() for _ in range(3)]
+            assert cnt.counts() == (0, 3)
+    finally:
+        lib.tff_fused_set_graphs(1)
+    m.check()
+    for g in got:
+        for k, a, b in zip(NAMES, g, ref):
+            assert np.array_equal(a, b), k
