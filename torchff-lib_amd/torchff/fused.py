@@ -250,6 +250,10 @@ class FusedNonbondedPME(nn.Module):
         status, host, row = self._status_slot(dev)
         bd = self._box_device(box, io)
         cd = L.F32 if self.compute_dtype == torch.float32 else L.F64
+        if for_grad:
+            # counted before the call: a call that fails part-way has still written the workspace
+            self._bws_gen += 1
+            self._last_token = None
         rc = lib.tff_fused_energy_forces(L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms,
                                          bh, L.ptr(rp), L.ptr(col), self.cutoff, self.list_cutoff, self.alpha,
                                          self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
@@ -257,7 +261,6 @@ class FusedNonbondedPME(nn.Module):
                                          L.ptr(ws), ws.numel(), L.stream_of(c))
         L.check(rc, "FusedNonbondedPME")
         if for_grad:
-            self._bws_gen += 1
             capturing = torch.cuda.is_current_stream_capturing()
             self._last_token = None if capturing else (self._bws_gen, L.stream_of(c).value)
         self._record(row, "forward call")
@@ -296,14 +299,14 @@ class FusedNonbondedPME(nn.Module):
         bd = self._box_device(box, io)
         rp, col = self.excl_row_ptr, self.excl_col_indices
         entry = lib.tff_fused_energy_forces_backward_reuse if reuse else lib.tff_fused_energy_forces_backward
+        if not reuse:
+            self._bws_gen += 1  # rebuilt in place (counted before the call, as in the forward)
         rc = entry(
             L.dtype_code(c), cd, L.ptr(c), L.ptr(q), L.ptr(s), L.ptr(e), self.natoms, bh, L.ptr(rp), L.ptr(col),
             self.cutoff, self.list_cutoff, self.alpha, self.coul, *self.K, self.order, *(L.ptr(m) for m in self._mod),
             L.ptr(ge), L.ptr(gf), *(L.ptr(o) for o in outs), L.ptr(status), L.ptr(bd), L.ptr(host),
             L.ptr(ws), ws.numel(), stream)
         L.check(rc, "FusedNonbondedPME backward")
-        if not reuse:
-            self._bws_gen += 1  # rebuilt in place
         self._record(row, "backward call")
         self.backward_status = status
         return outs
