@@ -719,10 +719,18 @@ __device__ __forceinline__ float octet_sum(float v) {
   v += dpp_f<0x141>(v);  // row_half_mirror
   return v;
 }
+// float64: the same lane permutations on both 32-bit halves (VALU moves, no LDS crossbar round trip)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 __device__ __forceinline__ double octet_sum(double v) {
-  v += __shfl_xor(v, 1, TFF_WAVE);
-  v += __shfl_xor(v, 2, TFF_WAVE);
-  v += __shfl_xor(v, 4, TFF_WAVE);
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror (every lane of a quad holds the quad's sum by now)
   return v;
 }
 // sum over the 8 lanes sharing lane & 7
@@ -1007,7 +1015,7 @@ __device__ __forceinline__ float octet_xchg(float v) {
 }
 template <int B>
 __device__ __forceinline__ double octet_xchg(double v) {
-  return __shfl_xor(v, 1 << B, TFF_WAVE);
+  return dpp_d<B == 2 ? 0x141 : B == 1 ? 0x4E : 0xB1>(v);
 }
 template <bool STRIDE8, int B, typename T>
 __device__ __forceinline__ T xchg8(T v) {
